@@ -1,0 +1,177 @@
+/*
+ * sssi.h -- C ABI of libsssi, the MI355X (gfx950) acoustic finite-difference
+ * propagators (CPML and sponge boundaries, 2-D and 3-D) and the reverse-time
+ * migration built on them.
+ *
+ * Conventions
+ *   - Host arrays are FP64, column-major (z fastest), exactly what a MATLAB
+ *     MEX gateway receives from mxGetPr:
+ *       v(nz,nx)            v[j*nz + i]
+ *       source(nz,nx,nt)    source[(t*nx + j)*nz + i]
+ *       data(nx,nt)         data[t*nx + j]
+ *       snapshot(nz,nx,ns)  snapshot[(s*nx + j)*nz + i]
+ *       3-D v(nz,nx,ny)     v[(k*nx + j)*nz + i];  data(nx,ny,nt)
+ *   - Outputs are caller-allocated.  No pointer is kept after a call returns.
+ *   - Every call returns SSSI_OK (0) or a negative status; sssi_last_error()
+ *     returns a thread-local message for the last failure on that thread.
+ *   - Device arithmetic is FP32 (stencil, memory variables); damping profiles
+ *     are formed in FP64 on the host and rounded once.
+ *   - Calls are synchronous unless stated otherwise.  Handles are not shared
+ *     between threads without external locking; distinct handles are
+ *     independent.
+ */
+#ifndef SSSI_H
+#define SSSI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSSI_OK 0
+#define SSSI_EINVAL (-1)    /* bad argument (shape, range, NULL)          */
+#define SSSI_ENOMEM (-2)    /* device or host allocation failed          */
+#define SSSI_EDEVICE (-3)   /* HIP runtime error / no device              */
+#define SSSI_ESTATE (-4)    /* call not valid in the handle's state       */
+#define SSSI_EBUDGET (-5)   /* job does not fit the device memory budget  */
+#define SSSI_EUNSTABLE (-6) /* NaN/Inf detected in the wavefield          */
+
+#define SSSI_ABC_CPML 0
+#define SSSI_ABC_SPONGE 1
+
+/* Message for the last failing call on the calling thread ("" if none). */
+const char *sssi_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int sssi_version(void);
+/* Number of visible HIP devices (0 on a host without GPU). */
+int sssi_device_count(int *count);
+/* Select the HIP device used by subsequently created handles and calls. */
+int sssi_set_device(int device);
+
+/* ---------------------------------------------------------------------
+ * Drop-in entry points: same arguments, order and outputs as the MEX
+ * gateways they replace.  snapshot may be NULL (the gateway passes NULL
+ * when nlhs < 2).  nDiffOrder in 1..8, 0 <= nBoundary, 2*nBoundary <= nx
+ * (and ny), nBoundary <= nz, nt >= 1.
+ * ------------------------------------------------------------------- */
+
+/* Replaces fwdTimeCpmlFor2dAw_mex (src/fd_cmex/fwdTimeCpmlFor2dAw_mex.c:39;
+ * wrapper src/fd_cmex/fwdTimeCpmlFor2dAw.m:37).  source is dense (nz,nx,nt).
+ * data (nx,nt), snapshot (nz,nx,nt). */
+int sssi_fwdTimeCpmlFor2dAw(const double *v, int64_t nz, int64_t nx, const double *source,
+                            int64_t nt, int nDiffOrder, int nBoundary, double dz, double dx,
+                            double dt, double *data, double *snapshot);
+
+/* Replaces rvsTimeCpmlFor2dAw_mex (src/fd_cmex/rvsTimeCpmlFor2dAw_mex.c:38).
+ * data (nx,nt) is injected at the top row for t = nt-1 ... 0.
+ * model ((nz+2l),(nx+2l)), l = 2*nDiffOrder-1, is the field one step before the
+ * last (rvs…_mex.c:285-287).  snapshot (nz,nx,nt).  model may be NULL. */
+int sssi_rvsTimeCpmlFor2dAw(const double *v, int64_t nz, int64_t nx, const double *data,
+                            int64_t nt, int nDiffOrder, int nBoundary, double dz, double dx,
+                            double dt, double *model, double *snapshot);
+
+/* Replaces fwdTimeCpmlFor3dAw (src/fwdTimeCpmlFor3dAw.m:1), including its
+ * damping profile conventions (fwdTimeCpmlFor3dAw.m:46-70).  source dense
+ * (nz,nx,ny,nt); data (nx,ny,nt); snapshot (nz,nx,ny,nt). */
+int sssi_fwdTimeCpmlFor3dAw(const double *v, int64_t nz, int64_t nx, int64_t ny,
+                            const double *source, int64_t nt, int nDiffOrder, int nBoundary,
+                            double dz, double dx, double dy, double dt, double *data,
+                            double *snapshot);
+
+/* Replaces fwdTimeSpongeFor2dAw (src/fwdTimeSpongeFor2dAw.m:1). */
+int sssi_fwdTimeSpongeFor2dAw(const double *v, int64_t nz, int64_t nx, const double *source,
+                              int64_t nt, int nBoundary, double dz, double dx, double dt,
+                              double *data, double *snapshot);
+
+/* Replaces rvsTimeSpongeFor2dAw (src/rvsTimeSpongeFor2dAw.m:1).
+ * model ((nz+2),(nx+2)) may be NULL. */
+int sssi_rvsTimeSpongeFor2dAw(const double *v, int64_t nz, int64_t nx, const double *data,
+                              int64_t nt, int nBoundary, double dz, double dx, double dt,
+                              double *model, double *snapshot);
+
+/* ---------------------------------------------------------------------
+ * Signature-preserving extensions used by the gateways when the dense
+ * source or the full snapshot stack would not fit (a 1000x3000 grid with
+ * 4000 steps needs ~100 GB for each): a point-source list with one wavelet
+ * column per point, and a snapshot stride (slice s holds step s*stride).
+ * ------------------------------------------------------------------- */
+
+/* Point sources: src_iz/src_ix (0-based, nsrc each), wavelet[t*nsrc + s]
+ * (nt x nsrc).  Identical arithmetic to a dense source that is zero away
+ * from the points (coincident points add).  snapshot holds ceil(nt/stride)
+ * slices, or is NULL. */
+int sssi_fwd_cpml_2d_points(const double *v, int64_t nz, int64_t nx, int64_t nt, int nsrc,
+                            const int *src_iz, const int *src_ix, const double *wavelet,
+                            int nDiffOrder, int nBoundary, double dz, double dx, double dt,
+                            double *data, double *snapshot, int64_t snap_stride);
+
+int sssi_rvs_cpml_2d_ex(const double *v, int64_t nz, int64_t nx, const double *data, int64_t nt,
+                        int nDiffOrder, int nBoundary, double dz, double dx, double dt,
+                        double *model, double *snapshot, int64_t snap_stride);
+
+int sssi_fwd_cpml_3d_points(const double *v, int64_t nz, int64_t nx, int64_t ny, int64_t nt,
+                            int nsrc, const int *src_iz, const int *src_ix, const int *src_iy,
+                            const double *wavelet, int nDiffOrder, int nBoundary, double dz,
+                            double dx, double dy, double dt, double *data, double *snapshot,
+                            int64_t snap_stride);
+
+int sssi_fwd_sponge_2d_points(const double *v, int64_t nz, int64_t nx, int64_t nt, int nsrc,
+                              const int *src_iz, const int *src_ix, const double *wavelet,
+                              int nBoundary, double dz, double dx, double dt, double *data,
+                              double *snapshot, int64_t snap_stride);
+
+int sssi_rvs_sponge_2d_ex(const double *v, int64_t nz, int64_t nx, const double *data,
+                          int64_t nt, int nBoundary, double dz, double dx, double dt,
+                          double *model, double *snapshot, int64_t snap_stride);
+
+/* ---------------------------------------------------------------------
+ * Device-resident propagation handle (2-D and 3-D).  Created once per
+ * model; inputs stay in HBM between runs so repeated propagations (shots,
+ * benchmark passes) pay no host transfer.  Runs are asynchronous on the
+ * handle's own HIP stream until sssi_prop_sync / a download.
+ * ------------------------------------------------------------------- */
+typedef struct sssi_prop sssi_prop;
+
+/* 2-D handle.  abc = SSSI_ABC_CPML or SSSI_ABC_SPONGE (nDiffOrder ignored
+ * for the sponge).  v column-major (nz,nx). */
+int sssi_prop2d_create(const double *v, int64_t nz, int64_t nx, int abc, int nDiffOrder,
+                       int nBoundary, double dz, double dx, double dt, sssi_prop **out);
+/* 3-D CPML handle, v column-major (nz,nx,ny). */
+int sssi_prop3d_create(const double *v, int64_t nz, int64_t nx, int64_t ny, int nDiffOrder,
+                       int nBoundary, double dz, double dx, double dy, double dt,
+                       sssi_prop **out);
+void sssi_prop_destroy(sssi_prop *p);
+
+/* Point sources (2-D: src_iy ignored, may be NULL). wavelet[t*nsrc + s]. */
+int sssi_prop_set_points(sssi_prop *p, int nsrc, const int *src_iz, const int *src_ix,
+                         const int *src_iy, const double *wavelet, int64_t nt);
+/* Surface data (nx,nt) injected at the top row on reverse runs (2-D). */
+int sssi_prop_set_surface(sssi_prop *p, const double *data, int64_t nt);
+/* Zero the wavefield and memory variables. */
+int sssi_prop_reset(sssi_prop *p);
+/* Run nt steps: forward (reverse = 0; point sources, t = 0..nt-1) or reverse
+ * (reverse = 1; surface data, t = nt-1..0).  Receivers (top row) are
+ * recorded on device when record != 0.  Asynchronous. */
+int sssi_prop_run(sssi_prop *p, int64_t nt, int reverse, int record);
+/* Same as sssi_prop_run, timed with HIP events on the handle's stream;
+ * returns elapsed milliseconds of the whole launch sequence and the
+ * number of kernel launches it made.  Synchronous. */
+int sssi_prop_run_timed(sssi_prop *p, int64_t nt, int reverse, int record, double *ms,
+                        int64_t *launches);
+int sssi_prop_sync(sssi_prop *p);
+/* Receiver traces of the last recorded run, column-major (nx,nt) for 2-D,
+ * (nx,ny,nt) for 3-D. */
+int sssi_prop_get_data(sssi_prop *p, double *data, int64_t nt);
+/* Current wavefield interior (nz,nx[,ny]) column-major, and the previous
+ * one padded as the reference's model output (2-D). */
+int sssi_prop_get_field(sssi_prop *p, double *field);
+int sssi_prop_get_model(sssi_prop *p, double *model);
+/* Whole-model statistics of the grid (cells, strip cells) for roofline
+ * accounting: algorithmic bytes per step of the propagation kernel. */
+int sssi_prop_alg_bytes_per_step(sssi_prop *p, double *bytes, double *cells);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SSSI_H */

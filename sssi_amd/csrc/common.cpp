@@ -1,0 +1,105 @@
+// common.cpp -- host helpers shared by the propagators (see common.h).
+#include "common.h"
+
+#include <cstring>
+
+namespace sssi {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const char *msg) { g_last_error = msg ? msg : ""; }
+
+const char *last_error_text() { return g_last_error.c_str(); }
+
+static int64_t gcd64(int64_t a, int64_t b) {
+    if (a < 0) a = -a;
+    if (b < 0) b = -b;
+    while (b) {
+        const int64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+void staggered_coefficients(int N, double *c) {
+    if (N < 1 || N > 8) fail(SSSI_EINVAL, "nDiffOrder must be in 1..8 (got %d)", N);
+    for (int j = 1; j <= N; ++j) {
+        int64_t num = 1, den = 2 * j - 1;
+        bool neg = false;
+        for (int m = 1; m <= N; ++m) {
+            if (m == j) continue;
+            const int64_t a = (int64_t)(2 * m - 1) * (2 * m - 1);
+            int64_t b = a - (int64_t)(2 * j - 1) * (2 * j - 1);
+            if (b < 0) { neg = !neg; b = -b; }
+            const int64_t g1 = gcd64(a, den), g2 = gcd64(num, b);
+            num = (num / g2) * (a / g1);
+            den = (den / g1) * (b / g2);
+            const int64_t g = gcd64(num, den);
+            num /= g;
+            den /= g;
+        }
+        c[j - 1] = (neg ? -1.0 : 1.0) * ((double)num / (double)den);
+    }
+}
+
+// fwdTimeCpmlFor2dAw_mex.c:100-145 with dampPml (finiteDifference.c:286-313):
+// d0 = -(3 v)/(2 L) log(1e-6), d = d0 (u/L)(u/L), b = exp(-d dt).
+void cpml2d_bm1(const double *v, int64_t nz, int64_t nx, int nB, double dz, double dx, double dt,
+                std::vector<float> &zbm1, std::vector<float> &xbm1) {
+    const double logR = std::log(1e-6);
+    zbm1.assign((size_t)nB * nx, 0.f);
+    xbm1.assign((size_t)nz * 2 * nB, 0.f);
+    if (nB == 0) return;
+    {
+        const double L = nB * dx;
+        for (int j = 0; j < nB; ++j)
+            for (int64_t i = 0; i < nz; ++i) {
+                const double u = (nB - j) * dx;
+                const double d0 = -(3.0 * v[(int64_t)j * nz + i]) / (2 * L) * logR;
+                const double d = d0 * (u / L) * (u / L);
+                xbm1[(size_t)i * 2 * nB + j] = (float)(std::exp(-d * dt) - 1.0);
+            }
+        for (int j = 0; j < nB; ++j)
+            for (int64_t i = 0; i < nz; ++i) {
+                const int64_t jj = nx - nB + j;
+                const double u = (j + 1) * dx;
+                const double d0 = -(3.0 * v[jj * nz + i]) / (2 * L) * logR;
+                const double d = d0 * (u / L) * (u / L);
+                xbm1[(size_t)i * 2 * nB + nB + j] = (float)(std::exp(-d * dt) - 1.0);
+            }
+    }
+    {
+        const double L = nB * dz;
+        for (int64_t j = 0; j < nx; ++j)
+            for (int i = 0; i < nB; ++i) {
+                const int64_t ii = nz - nB + i;
+                const double u = (i + 1) * dz;
+                const double d0 = -(3.0 * v[j * nz + ii]) / (2 * L) * logR;
+                const double d = d0 * (u / L) * (u / L);
+                zbm1[(size_t)i * nx + j] = (float)(std::exp(-d * dt) - 1.0);
+            }
+    }
+}
+
+void validate_common(int64_t nz, int64_t nx, int64_t ny, int64_t nt, int N, int nB, double dz,
+                     double dx, double dy, double dt, bool need_N) {
+    if (nz < 1 || nx < 1 || ny < 1)
+        fail(SSSI_EINVAL, "grid must be non-empty (nz=%lld nx=%lld ny=%lld)", (long long)nz,
+             (long long)nx, (long long)ny);
+    if (nz > (1 << 24) || nx > (1 << 24) || ny > (1 << 24) || nz * nx > (int64_t)1 << 30)
+        fail(SSSI_EINVAL, "grid too large for the 2-D index space");
+    if (nt < 1) fail(SSSI_EINVAL, "nt must be >= 1 (got %lld)", (long long)nt);
+    if (need_N && (N < 1 || N > 8)) fail(SSSI_EINVAL, "nDiffOrder must be in 1..8 (got %d)", N);
+    if (nB < 0) fail(SSSI_EINVAL, "nBoundary must be >= 0 (got %d)", nB);
+    if ((int64_t)nB > nz) fail(SSSI_EINVAL, "nBoundary=%d exceeds nz=%lld", nB, (long long)nz);
+    if (2 * (int64_t)nB > nx)
+        fail(SSSI_EINVAL, "2*nBoundary=%d exceeds nx=%lld", 2 * nB, (long long)nx);
+    if (ny > 1 && 2 * (int64_t)nB > ny)
+        fail(SSSI_EINVAL, "2*nBoundary=%d exceeds ny=%lld", 2 * nB, (long long)ny);
+    if (!(dz > 0) || !(dx > 0) || !(dy > 0) || !(dt > 0) || !std::isfinite(dz) ||
+        !std::isfinite(dx) || !std::isfinite(dy) || !std::isfinite(dt))
+        fail(SSSI_EINVAL, "dz, dx, dy and dt must be positive and finite");
+}
+
+}  // namespace sssi

@@ -1,0 +1,92 @@
+// common.h -- shared host helpers of libsssi: status codes, thread-local error
+// text, HIP error checks, device buffers, the staggered coefficients and the
+// CPML damping profiles (formed in FP64 on the host, rounded once to FP32).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sssi.h"
+
+namespace sssi {
+
+// Exception carrying an ABI status code; converted at the C boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const char *msg);
+const char *last_error_text();
+
+[[noreturn]] inline void fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    throw Error(code, buf);
+}
+
+#define SSSI_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            ::sssi::fail(e_ == hipErrorOutOfMemory ? SSSI_ENOMEM : SSSI_EDEVICE,        \
+                         "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),         \
+                         __FILE__, __LINE__);                                           \
+    } while (0)
+
+// Owning device buffer.
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) SSSI_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Staggered coefficients c_j (dCoef(N,'s'), finiteDifference.c:57-115): exact
+// rational c_j = (1/(2j-1)) prod_{m!=j} (2m-1)^2/((2m-1)^2-(2j-1)^2), reduced
+// and rounded once (numerator and denominator stay below 2^53 for N <= 8).
+void staggered_coefficients(int N, double *c);
+
+// 2-D CPML b-profiles as in fwdTimeCpmlFor2dAw_mex.c:100-145 (FP64), returned
+// as b-1 for the strips only (compact layouts):
+//   zbm1[(i-(nz-nB))*nx + j]   rows nz-nB..nz-1, all columns
+//   xbm1[i*2nB + xs(j)]        columns 0..nB-1 and nx-nB..nx-1, all rows
+void cpml2d_bm1(const double *v, int64_t nz, int64_t nx, int nB, double dz, double dx, double dt,
+                std::vector<float> &zbm1, std::vector<float> &xbm1);
+
+inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+void validate_common(int64_t nz, int64_t nx, int64_t ny, int64_t nt, int N, int nB, double dz,
+                     double dx, double dy, double dt, bool need_N);
+
+}  // namespace sssi

@@ -1,0 +1,77 @@
+// prop.h -- device-resident propagation handles (2-D CPML/sponge, 3-D CPML).
+#pragma once
+
+#include "common.h"
+
+struct sssi_prop;  // opaque ABI handle; defined as a subclass of sssi::Prop
+
+namespace sssi {
+
+// Padded device layout of a 2-D wavefield: interior cell (i, j) lives at
+// (i + kHZ) * pitch + (j + kHX).  The halo is zero and never written, so the
+// stencil reads past the interior without bounds checks (the reference's
+// zero padding of width l = 2N-1, N <= 8).
+constexpr int kHZ = 16;
+constexpr int kHX = 32;
+constexpr int kTX = 64;     // tile width (one wavefront across x)
+constexpr int kTZmax = 64;  // tallest tile the kernels use
+
+struct LaunchStats {
+    int64_t launches = 0;
+};
+
+// Where snapshots go during a run: device buffer of compact FP32 slices
+// (x fastest), one per step t with t % stride == 0, at slot t/stride - slot0.
+struct SnapSink {
+    float *dev = nullptr;
+    int64_t stride = 1;
+    int64_t slot0 = 0;
+};
+
+class Prop {
+   public:
+    virtual ~Prop() = default;
+    int dim = 2;
+    int abc = SSSI_ABC_CPML;
+    int64_t nz = 0, nx = 0, ny = 1;
+    int N = 1, nB = 0;
+    double dz = 0, dx = 0, dy = 0, dt = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // point sources, sorted by tile; amplitudes [t][k] in sorted order
+    int nsrc = 0;
+    int64_t src_nt = 0;
+    DevBuf<int> src_ptr, src_cell;
+    DevBuf<float> src_amp;
+    // surface data for reverse runs: [t][j]
+    int64_t surf_nt = 0;
+    DevBuf<float> surf;
+    // receivers: [t][j] (2-D) or [t][k][j] (3-D)
+    int64_t rec_nt = 0;
+    DevBuf<float> rec;
+
+    virtual void reset() = 0;
+    virtual void set_points(int n, const int *iz, const int *ix, const int *iy,
+                            const double *wavelet, int64_t nt) = 0;
+    virtual void set_surface(const double *data, int64_t nt) = 0;
+    // Enqueue `count` steps starting at time index t_first: forward runs use
+    // t = t_first, t_first+1, ...; reverse runs t = t_first, t_first-1, ...
+    // Point sources drive forward runs, surface data reverse runs.
+    virtual void run(int64_t t_first, int64_t count, bool reverse, bool record,
+                     const SnapSink *snap, LaunchStats *st) = 0;
+    virtual int64_t cells() const = 0;
+    virtual void get_data(double *data, int64_t nt) = 0;
+    virtual void get_field(double *field) = 0;
+    virtual void get_model(double *model) = 0;
+    virtual void alg_bytes(double *bytes, double *cells) = 0;
+
+    void sync() { SSSI_HIP(hipStreamSynchronize(stream)); }
+};
+
+Prop *make_prop2d(const double *v, int64_t nz, int64_t nx, int abc, int N, int nB, double dz,
+                  double dx, double dt);
+Prop *make_prop3d(const double *v, int64_t nz, int64_t nx, int64_t ny, int N, int nB,
+                  double dz, double dx, double dy, double dt);
+
+}  // namespace sssi
