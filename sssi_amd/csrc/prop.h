@@ -12,6 +12,8 @@ namespace sssi {
 // stencil reads past the interior without bounds checks (the reference's
 // zero padding of width l = 2N-1, N <= 8).
 constexpr int kHZ = 16;
+constexpr int kHZbot = 32;  // rows below the (rounded-up) interior: the z sweep
+                            // prefetches up to RZ + L - 1 rows past the last row
 constexpr int kHX = 32;
 constexpr int kTX = 64;     // tile width (one wavefront across x)
 constexpr int kTZmax = 64;  // tallest tile the kernels use

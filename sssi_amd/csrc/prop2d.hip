@@ -19,6 +19,7 @@
 // points.  Tiles are numbered so that each of the 8 XCDs gets a contiguous
 // band (halo rows shared by neighbouring tiles stay in that XCD's L2).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -30,15 +31,17 @@ struct Cpml2dArgs {
     const float *pcur;
     float *pold;  // read p_old, write p_new
     const float *vdt2;
-    int nz, nx, pitch, ntx, nblk, nB;
-    float cz[8], cx[8];
+    int nz, nx, pitch, rows, ntx, nblk, nB;
+    const int *order;           // block -> tile (strip tiles first, XCD-banded interior)
+    float cz[8], cx[8];         // c_k/dz, c_k/dx (two-stage form)
+    float w0, wz[16], wx[16];   // composite D-D+ weights / d^2 (w0: both axes)
     const float *zphi_in;
     float *zphi_out, *zpsi;
     const float *zbm1;
     const float *xphi_in;
     float *xphi_out, *xpsi;
     const float *xbm1;
-    const int *src_ptr;   // per tile [ptr, ptr+1) into src_cell/src_amp, or null
+    const int *src_ptr;   // per wave tile [ptr, ptr+1) into src_cell/src_amp, or null
     const int *src_cell;  // i * nx + j
     const float *src_amp; // amplitudes of this step (sorted order)
     const float *surf;    // this step's surface row (reverse runs) or null
@@ -53,131 +56,228 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk) {
     return x * q + (x < r ? x : r) + k;
 }
 
+// LDS written by some lanes of a wavefront and read by others: wavefront-scope
+// release/acquire (no workgroup barrier; waves of a block are independent).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Raw buffer access: the wave-uniform row offset goes in the scalar offset,
+// the lane's column in the vector offset, so a load costs no vector address
+// arithmetic.  Out-of-range offsets (beyond num_records) read 0 and drop stores.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                       uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+}
+
+// One time step.  Each wavefront owns a 64 x RZ tile (lane = column); the four
+// wavefronts of a block take four consecutive tiles in z and never
+// synchronise with each other.  All global loads of a tile (z window, x halo,
+// p_old, v^2dt^2) are issued before the first use.  Tiles whose stencil reach
+// touches no damping strip use the composite (D-D+) stencil of radius
+// L = 2N-1, which equals the two-stage form there (Phi = Psi = 0); strip
+// tiles run the two-stage CPML update.
 template <int N, int RZ>
 __global__ void __launch_bounds__(256) cpml2d_step(Cpml2dArgs a) {
     constexpr int L = 2 * N - 1;
-    constexpr int TZ = 4 * RZ;
-    constexpr int PW = kTX + 2 * L;  // staged p tile: columns x0-L .. x0+64+L-1
-    constexpr int PH = TZ + 2 * L;   //                rows    z0-L .. z0+TZ+L-1
-    constexpr int AW = kTX + L;      // A_x half points x0-N .. x0+64+N-2
-    __shared__ float sp[PH * PW];
-    __shared__ float sa[TZ * AW];
+    constexpr int PW = kTX + 2 * L;  // p rows: columns x0-L .. x0+63+L
+    constexpr int AW = kTX + L;      // A_x rows: half points x0-N .. x0+63+N-1
+    constexpr int HN = 2 * L * RZ;   // x-halo values per tile
+    constexpr int HPL = (HN + 63) / 64;
+    __shared__ float s_p[4][RZ * PW];
+    __shared__ float s_a[4][RZ * AW];
 
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int tile = xcd_tile(blockIdx.x, a.nblk);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tile = a.order[blockIdx.x];
     const int tzi = tile / a.ntx, txi = tile - tzi * a.ntx;
-    const int x0 = txi * kTX, z0 = tzi * TZ;
+    const int x0 = txi * kTX;
     const int nz = a.nz, nx = a.nx, nB = a.nB;
-    const ptrdiff_t pitch = a.pitch;
-    const float *P = a.pcur + kHZ * pitch + kHX;
-
-    for (int e = tid; e < PH * PW; e += 256) {
-        const int r = e / PW, c = e - r * PW;
-        sp[e] = P[(ptrdiff_t)(z0 - L + r) * pitch + (x0 - L + c)];
-    }
-    __syncthreads();
-
+    const int zs = (tzi * 4 + w) * RZ;
+    if (zs >= nz) return;
+    const uint32_t rowb = (uint32_t)a.pitch * 4u;
+    const uint32_t nbytes = rowb * (uint32_t)a.rows;
+    const __amdgpu_buffer_rsrc_t rP = buf_rsrc(a.pcur, nbytes);
+    const __amdgpu_buffer_rsrc_t rO = buf_rsrc(a.pold, nbytes);
+    const __amdgpu_buffer_rsrc_t rV = buf_rsrc(a.vdt2, nbytes);
     const int j = x0 + lane;
-    const int zr0 = w * RZ;
-    const int zs = z0 + zr0;
     const bool jin = j < nx;
+    const uint32_t vcol = (uint32_t)(j + kHX) * 4u;
+    const uint32_t srow = (uint32_t)(zs + kHZ) * rowb;  // first row of the tile
 
-    // ---- z: column window p(zs-L .. zs+RZ+L-1) from LDS into registers
-    float col[RZ + 2 * L];
+    // ---- issue every global load of the tile
+    float col[RZ + 2 * L], po[RZ], v2[RZ], hal[HPL];
 #pragma unroll
-    for (int r = 0; r < RZ + 2 * L; ++r) col[r] = sp[(zr0 + r) * PW + lane + L];
-
-    // A_z at half points h = zs-N+q (q < RZ+L): D+p(h) = sum c_k (p(h+1+k) - p(h-k))
-    float Az[RZ + L];
+    for (int r = 0; r < RZ + 2 * L; ++r) col[r] = buf_ld(rP, vcol, srow + (r - L) * rowb);
 #pragma unroll
-    for (int q = 0; q < RZ + L; ++q) {
-        float d = 0.f;
-#pragma unroll
-        for (int k = 0; k < N; ++k) d = fmaf(a.cz[k], col[q + N + k] - col[q + N - 1 - k], d);
-        Az[q] = d;
+    for (int r = 0; r < RZ; ++r) {
+        po[r] = buf_ld(rO, vcol, srow + r * rowb);
+        v2[r] = buf_ld(rV, vcol, srow + r * rowb);
     }
+    int h_lds[HPL];
+#pragma unroll
+    for (int u = 0; u < HPL; ++u) {
+        const int e = lane + 64 * u;
+        const int r = e / (2 * L), c = e - r * (2 * L);
+        const int xc = c < L ? x0 - L + c : x0 + kTX - L + c;
+        h_lds[u] = e < HN ? r * PW + (c < L ? c : kTX + c) : -1;
+        hal[u] = buf_ld(rP, (uint32_t)(r * a.pitch + xc + kHX) * 4u, srow);
+    }
+
+    // ---- the tile's rows with an L-wide x halo into the wave's LDS rows
+    float *sp = s_p[w];
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) sp[r * PW + L + lane] = col[L + r];
+#pragma unroll
+    for (int u = 0; u < HPL; ++u)
+        if (h_lds[u] >= 0) sp[h_lds[u]] = hal[u];
+    wave_lds_sync();
+
     const int zb0 = nz - nB;
-    const bool zstrip = nB > 0 && (z0 + TZ + N - 1 > zb0);
-    if (zstrip && jin) {
+    const bool zstrip = nB > 0 && (zs + RZ + N - 1 > zb0);
+    const bool xstrip = nB > 0 && (x0 - N < nB || x0 + kTX + N - 1 > nx - nB);
+    float lap[RZ];
+    if (!zstrip && !xstrip) {
+        // ---- interior: composite stencil
+#pragma unroll
+        for (int r = 0; r < RZ; ++r) {
+            const float *row = sp + r * PW + L + lane;
+            float acc = a.w0 * col[L + r];
+#pragma unroll
+            for (int m = 1; m <= L; ++m) acc = fmaf(a.wz[m], col[L + r + m] + col[L + r - m], acc);
+#pragma unroll
+            for (int m = 1; m <= L; ++m) acc = fmaf(a.wx[m], row[m] + row[-m], acc);
+            lap[r] = acc;
+        }
+    } else {
+        // ---- damping strips: two-stage form with Phi/Psi
+        // A_z at half points h = zs-N+q (q < RZ+L): D+p(h) = sum c_k (p(h+1+k) - p(h-k))
+        float Az[RZ + L];
 #pragma unroll
         for (int q = 0; q < RZ + L; ++q) {
-            const int h = zs - N + q;
-            if (h >= zb0 && h < nz) {
-                const size_t o = (size_t)(h - zb0) * nx + j;
-                float phi = a.zphi_in[o];
-                phi = fmaf(a.zbm1[o], phi + Az[q], phi);  // b*phi + (b-1)*D
-                Az[q] += phi;
-                if (h >= zs && h < zs + RZ) a.zphi_out[o] = phi;
+            float d = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) d = fmaf(a.cz[k], col[q + N + k] - col[q + N - 1 - k], d);
+            Az[q] = d;
+        }
+        if (zstrip && jin) {
+            // all z-strip memory variables of the tile in one batch of loads
+            // (b-1) at half point h and at row i = h share one index
+            float phz[RZ + L], bmh[RZ + L], psz[RZ];
+#pragma unroll
+            for (int q = 0; q < RZ + L; ++q) {
+                const int h = zs - N + q;
+                const bool in = h >= zb0 && h < nz;
+                const size_t o = in ? (size_t)(h - zb0) * nx + j : 0;
+                phz[q] = in ? a.zphi_in[o] : 0.f;
+                bmh[q] = in ? a.zbm1[o] : 0.f;
+            }
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) {
+                const int i = zs + r;
+                const bool in = i >= zb0 && i < nz;
+                psz[r] = in ? a.zpsi[(size_t)(i - zb0) * nx + j] : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < RZ + L; ++q) {
+                const int h = zs - N + q;
+                if (h >= zb0 && h < nz) {
+                    const float phi = fmaf(bmh[q], phz[q] + Az[q], phz[q]);  // b*phi + (b-1)*D
+                    Az[q] += phi;
+                    if (h >= zs && h < zs + RZ) a.zphi_out[(size_t)(h - zb0) * nx + j] = phi;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) {
+                float d = 0.f;
+#pragma unroll
+                for (int k = 0; k < N; ++k) d = fmaf(a.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
+                const int i = zs + r;
+                if (i >= zb0 && i < nz) {
+                    const float psi = fmaf(bmh[r + N], psz[r] + d, psz[r]);
+                    a.zpsi[(size_t)(i - zb0) * nx + j] = psi;
+                    d += psi;
+                }
+                lap[r] = d;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) {
+                float d = 0.f;
+#pragma unroll
+                for (int k = 0; k < N; ++k) d = fmaf(a.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
+                lap[r] = d;
             }
         }
-    }
-    // P_z = D-A_z + Psi_z
-    float Pz[RZ];
-#pragma unroll
-    for (int r = 0; r < RZ; ++r) {
-        float d = 0.f;
-#pragma unroll
-        for (int k = 0; k < N; ++k) d = fmaf(a.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
-        Pz[r] = d;
-    }
-    if (zstrip && jin) {
-#pragma unroll
-        for (int r = 0; r < RZ; ++r) {
-            const int i = zs + r;
-            if (i >= zb0 && i < nz) {
-                const size_t o = (size_t)(i - zb0) * nx + j;
-                float psi = a.zpsi[o];
-                psi = fmaf(a.zbm1[o], psi + Pz[r], psi);
-                a.zpsi[o] = psi;
-                Pz[r] += psi;
-            }
-        }
-    }
-
-    // ---- x: A_x at half points h = x0-N+q for every tile row, into LDS
-    const bool xstrip = nB > 0 && (x0 - N < nB || x0 + kTX + N - 1 > nx - nB);
-    const int nb2 = 2 * nB;
-    for (int e = tid; e < TZ * AW; e += 256) {
-        const int rr = e / AW, q = e - rr * AW;
-        const float *row = sp + (rr + L) * PW;
-        float d = 0.f;
-#pragma unroll
-        for (int k = 0; k < N; ++k) d = fmaf(a.cx[k], row[q + N + k] - row[q + N - 1 - k], d);
-        if (xstrip) {
-            const int h = x0 - N + q, i = z0 + rr;
-            if (i < nz && h >= 0 && h < nx && (h < nB || h >= nx - nB)) {
-                const size_t o = (size_t)i * nb2 + (h < nB ? h : h - (nx - nb2));
-                float phi = a.xphi_in[o];
-                phi = fmaf(a.xbm1[o], phi + d, phi);
-                d += phi;
-                if (h >= x0 && h < x0 + kTX) a.xphi_out[o] = phi;
-            }
-        }
-        sa[e] = d;
-    }
-    __syncthreads();
-
-    float Px[RZ];
-#pragma unroll
-    for (int r = 0; r < RZ; ++r) {
-        const float *ar = sa + (zr0 + r) * AW + lane + N;
-        float d = 0.f;
-#pragma unroll
-        for (int k = 0; k < N; ++k) d = fmaf(a.cx[k], ar[k] - ar[-1 - k], d);
-        Px[r] = d;
-    }
-    if (xstrip && jin && (j < nB || j >= nx - nB)) {
+        // A_x at half points h = x0-N+q, q < 64+L: lane q, lanes < L also q = 64+lane
+        float *sa = s_a[w];
+        const int nb2 = 2 * nB;
+        const bool xown = xstrip && jin && (j < nB || j >= nx - nB);
         const int xs = j < nB ? j : j - (nx - nb2);
+        // all x-strip memory variables of the tile in one batch of loads
+        float phx[2][RZ], bmx[2][RZ], psx[RZ], bmo[RZ];
 #pragma unroll
         for (int r = 0; r < RZ; ++r) {
             const int i = zs + r;
-            if (i < nz) {
-                const size_t o = (size_t)i * nb2 + xs;
-                float psi = a.xpsi[o];
-                psi = fmaf(a.xbm1[o], psi + Px[r], psi);
-                a.xpsi[o] = psi;
-                Px[r] += psi;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int q = lane + half * kTX;
+                const int h = x0 - N + q;
+                const bool in = xstrip && (half == 0 || lane < L) && i < nz && h >= 0 && h < nx &&
+                                (h < nB || h >= nx - nB);
+                const size_t o = in ? (size_t)i * nb2 + (h < nB ? h : h - (nx - nb2)) : 0;
+                phx[half][r] = in ? a.xphi_in[o] : 0.f;
+                bmx[half][r] = in ? a.xbm1[o] : 0.f;
             }
+            const bool own = xown && i < nz;
+            const size_t o = own ? (size_t)i * nb2 + xs : 0;
+            psx[r] = own ? a.xpsi[o] : 0.f;
+            bmo[r] = own ? a.xbm1[o] : 0.f;
+        }
+#pragma unroll
+        for (int r = 0; r < RZ; ++r) {
+            const float *row = sp + r * PW;
+            const int i = zs + r;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int q = lane + half * kTX;
+                if (half == 1 && lane >= L) break;
+                float d = 0.f;
+#pragma unroll
+                for (int k = 0; k < N; ++k)
+                    d = fmaf(a.cx[k], row[q + N + k] - row[q + N - 1 - k], d);
+                const int h = x0 - N + q;
+                if (xstrip && i < nz && h >= 0 && h < nx && (h < nB || h >= nx - nB)) {
+                    const float phi = fmaf(bmx[half][r], phx[half][r] + d, phx[half][r]);
+                    d += phi;
+                    if (h >= x0 && h < x0 + kTX)
+                        a.xphi_out[(size_t)i * nb2 + (h < nB ? h : h - (nx - nb2))] = phi;
+                }
+                sa[r * AW + q] = d;
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < RZ; ++r) {
+            const float *ar = sa + r * AW + lane + N;
+            float d = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) d = fmaf(a.cx[k], ar[k] - ar[-1 - k], d);
+            const int i = zs + r;
+            if (xown && i < nz) {
+                const float psi = fmaf(bmo[r], psx[r] + d, psx[r]);
+                a.xpsi[(size_t)i * nb2 + xs] = psi;
+                d += psi;
+            }
+            lap[r] += d;
         }
     }
 
@@ -186,7 +286,8 @@ __global__ void __launch_bounds__(256) cpml2d_step(Cpml2dArgs a) {
 #pragma unroll
     for (int r = 0; r < RZ; ++r) s[r] = 0.f;
     if (a.src_ptr) {
-        const int k0 = a.src_ptr[tile], k1 = a.src_ptr[tile + 1];
+        const int wt = (zs / RZ) * a.ntx + txi;
+        const int k0 = a.src_ptr[wt], k1 = a.src_ptr[wt + 1];
         for (int k = k0; k < k1; ++k) {
             const int cell = a.src_cell[k];
             const int ci = cell / nx, cj = cell - ci * nx;
@@ -205,14 +306,12 @@ __global__ void __launch_bounds__(256) cpml2d_step(Cpml2dArgs a) {
     for (int r = 0; r < RZ; ++r) {
         const int i = zs + r;
         if (i >= nz) break;
-        const size_t g = (size_t)(i + kHZ) * pitch + (j + kHX);
         const float pc = col[r + L];
-        const float po = a.pold[g];
-        // pc + (v2*(...) + (pc - po)): pc - po is exact when the two levels
-        // are within a factor 2, which keeps the rounding of the update at
-        // the scale of the increment rather than of the field.
-        const float pn = pc + fmaf(a.vdt2[g], (Pz[r] + Px[r]) + s[r], pc - po);
-        a.pold[g] = pn;
+        // pc + (v2*(...) + (pc - po)): pc - po is exact when the two levels are
+        // within a factor 2, which keeps the rounding of the update at the
+        // scale of the increment rather than of the field.
+        const float pn = pc + fmaf(v2[r], lap[r] + s[r], pc - po[r]);
+        buf_st(pn, rO, vcol, srow + r * rowb);
         if (a.rec && i == 0) a.rec[j] = pn;
         if (a.snap) a.snap[(size_t)i * nx + j] = pn;
     }
@@ -289,6 +388,23 @@ __global__ void unpad2d(const float *src, float *dst, int nz, int nx, int pitch,
     dst[(size_t)i * nx + j] = f * src[(size_t)(i + kHZ) * pitch + j + kHX];
 }
 
+// Composite weights of D-D+ (before division by d^2): w[L+m], m = -L..L, with
+// D+p(h) = sum_q c_q (p(h+1+q) - p(h-q)) and D-A(i) = sum_k c_k (A(i+k) - A(i-1-k)).
+static void composite_weights(int N, const double *c, double *w) {
+    const int L = 2 * N - 1;
+    for (int m = 0; m <= 2 * L; ++m) w[m] = 0.0;
+    for (int k = 0; k < N; ++k)
+        for (int q = 0; q < N; ++q) {
+            const double cc = c[k] * c[q];
+            w[L + k + 1 + q] += cc;   // A(i+k) <- p(i+k+1+q)
+            w[L + k - q] -= cc;       // A(i+k) <- p(i+k-q)
+            w[L - k + q] -= cc;       // A(i-1-k) <- p(i-k+q)
+            w[L - 1 - k - q] += cc;   // A(i-1-k) <- p(i-1-k-q)
+        }
+}
+
+static int default_rz(int N) { return N >= 6 ? 4 : 4; }
+
 // ---------------------------------------------------------------------------
 class Prop2D final : public Prop {
    public:
@@ -297,9 +413,11 @@ class Prop2D final : public Prop {
     DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1, gtab;
     int cur = 0, phase = 0;
     float cz[8] = {0}, cx[8] = {0};
-    int rz = 8;  // rows per wavefront (tile height 4*rz)
+    int rz = 8;  // rows per wavefront tile (64 x rz); a block holds 4 tiles stacked in z
+    float w0 = 0.f, wz[16] = {0}, wx[16] = {0};
     std::vector<int> h_src_cell;  // sponge: unsorted cell list
     DevBuf<float> snap_tmp;
+    DevBuf<int> order;  // launch position -> block tile
 
     Prop2D(const double *v, int64_t nz_, int64_t nx_, int abc_, int N_, int nB_, double dz_,
            double dx_, double dt_) {
@@ -309,7 +427,7 @@ class Prop2D final : public Prop {
         SSSI_HIP(hipGetDevice(&device));
         SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         pitch = round_up(kHX + round_up(nx, kTX) + 32, 64);
-        rows = kHZ + round_up(nz, kTZmax) + kHZ;
+        rows = kHZ + round_up(nz, kTZmax) + kHZbot;
         const size_t np = (size_t)pitch * rows;
         p[0].alloc(np);
         p[1].alloc(np);
@@ -329,6 +447,14 @@ class Prop2D final : public Prop {
                 cz[k] = (float)(c[k] / dz);
                 cx[k] = (float)(c[k] / dx);
             }
+            double w[32];
+            composite_weights(N, c, w);
+            const int L = 2 * N - 1;
+            w0 = (float)(w[L] / (dz * dz) + w[L] / (dx * dx));
+            for (int m = 1; m <= L; ++m) {
+                wz[m] = (float)(w[L + m] / (dz * dz));
+                wx[m] = (float)(w[L + m] / (dx * dx));
+            }
             if (nB > 0) {
                 std::vector<float> zb, xb;
                 cpml2d_bm1(v, nz, nx, nB, dz, dx, dt, zb, xb);
@@ -343,7 +469,12 @@ class Prop2D final : public Prop {
                 zpsi.alloc(zb.size());
                 xpsi.alloc(xb.size());
             }
-            rz = (N >= 6) ? 4 : 8;
+            rz = default_rz(N);
+            if (const char *e = std::getenv("SSSI_RZ")) {
+                const int r = std::atoi(e);
+                if (r == 4 || r == 8) rz = r;
+            }
+            build_order();
         } else {
             std::vector<float> g(std::max(nB, 1), 1.f);
             const double alpha = 0.015;
@@ -366,6 +497,43 @@ class Prop2D final : public Prop {
     int tz() const { return 4 * rz; }
     int ntx() const { return (int)((nx + kTX - 1) / kTX); }
     int ntz() const { return (int)((nz + tz() - 1) / tz()); }
+    int nwz() const { return (int)((nz + rz - 1) / rz); }  // wave tiles along z
+
+    // Launch order of the block tiles (64 x 4*rz).  Tiles that run the
+    // two-stage CPML update (damping strips within the stencil reach) are
+    // launched first so their longer dependency chains overlap the interior
+    // tiles instead of forming a tail.  Interior tiles follow, split into 8
+    // contiguous bands: block b is dispatched to XCD b % 8, so each XCD works
+    // through one band and the halo rows shared by neighbouring tiles stay in
+    // its L2.
+    void build_order() {
+        const int ntx_ = ntx(), ntz_ = ntz();
+        const int nblk = ntx_ * ntz_;
+        std::vector<int> strip, inner;
+        for (int tzi = 0; tzi < ntz_; ++tzi)
+            for (int txi = 0; txi < ntx_; ++txi) {
+                const int64_t z0 = (int64_t)tzi * tz(), x0 = (int64_t)txi * kTX;
+                const bool zs = nB > 0 && z0 + tz() + N - 1 > nz - nB;
+                const bool xs = nB > 0 && (x0 - N < nB || x0 + kTX + N - 1 > nx - nB);
+                (zs || xs ? strip : inner).push_back(tzi * ntx_ + txi);
+            }
+        std::vector<int> ord(strip);
+        const size_t ni = inner.size();
+        std::vector<size_t> next(8), end(8);
+        for (int x = 0; x < 8; ++x) {
+            next[x] = ni * x / 8;
+            end[x] = ni * (x + 1) / 8;
+        }
+        for (size_t b = ord.size(); b < (size_t)nblk; ++b) {
+            int x = (int)(b % 8);
+            if (next[x] == end[x])
+                for (int y = 0; y < 8; ++y)
+                    if (next[y] < end[y]) { x = y; break; }
+            ord.push_back(inner[next[x]++]);
+        }
+        order.alloc(ord.size());
+        SSSI_HIP(hipMemcpy(order.p, ord.data(), order.bytes(), hipMemcpyHostToDevice));
+    }
 
     void reset() override {
         SSSI_HIP(hipMemsetAsync(p[0].p, 0, p[0].bytes(), stream));
@@ -395,10 +563,11 @@ class Prop2D final : public Prop {
         const int ntx_ = ntx();
         std::vector<int> order(n);
         std::iota(order.begin(), order.end(), 0);
-        auto tile_of = [&](int k) { return (iz[k] / tz()) * ntx_ + ix[k] / kTX; };
+        // per wavefront tile (64 x rz), the unit the kernel scans
+        auto tile_of = [&](int k) { return (iz[k] / rz) * ntx_ + ix[k] / kTX; };
         std::stable_sort(order.begin(), order.end(),
                          [&](int a_, int b_) { return tile_of(a_) < tile_of(b_); });
-        const int ntiles = ntx_ * ntz();
+        const int ntiles = ntx_ * nwz();
         std::vector<int> ptr(ntiles + 1, 0), cell(n);
         for (int k = 0; k < n; ++k) {
             ptr[tile_of(order[k]) + 1]++;
@@ -432,8 +601,7 @@ class Prop2D final : public Prop {
 
     template <int NN, int RZ>
     void launch_cpml(const Cpml2dArgs &args) {
-        const int nblk = args.nblk;
-        hipLaunchKernelGGL((cpml2d_step<NN, RZ>), dim3(nblk), dim3(256), 0, stream, args);
+        hipLaunchKernelGGL((cpml2d_step<NN, RZ>), dim3(args.nblk), dim3(256), 0, stream, args);
     }
 
     void step_cpml(int64_t t, bool reverse, bool record, float *snap) {
@@ -441,11 +609,14 @@ class Prop2D final : public Prop {
         a.pcur = p[cur].p;
         a.pold = p[cur ^ 1].p;
         a.vdt2 = vfield.p;
-        a.nz = (int)nz; a.nx = (int)nx; a.pitch = (int)pitch;
+        a.nz = (int)nz; a.nx = (int)nx; a.pitch = (int)pitch; a.rows = (int)rows;
         a.ntx = ntx();
         a.nblk = ntx() * ntz();
+        a.order = order.p;
         a.nB = nB;
         for (int k = 0; k < 8; ++k) { a.cz[k] = cz[k]; a.cx[k] = cx[k]; }
+        a.w0 = w0;
+        for (int m = 0; m < 16; ++m) { a.wz[m] = wz[m]; a.wx[m] = wx[m]; }
         if (nB > 0) {
             a.zphi_in = zphi[phase].p; a.zphi_out = zphi[phase ^ 1].p; a.zpsi = zpsi.p;
             a.zbm1 = zbm1.p;
@@ -463,8 +634,10 @@ class Prop2D final : public Prop {
         const int key = N * 16 + rz;
         switch (key) {
 #define SSSI_CASE(NN, R) case NN * 16 + R: launch_cpml<NN, R>(a); break;
-            SSSI_CASE(1, 8) SSSI_CASE(2, 8) SSSI_CASE(3, 8) SSSI_CASE(4, 8) SSSI_CASE(5, 8)
-            SSSI_CASE(6, 4) SSSI_CASE(7, 4) SSSI_CASE(8, 4)
+            SSSI_CASE(1, 4) SSSI_CASE(1, 8) SSSI_CASE(2, 4) SSSI_CASE(2, 8)
+            SSSI_CASE(3, 4) SSSI_CASE(3, 8) SSSI_CASE(4, 4) SSSI_CASE(4, 8)
+            SSSI_CASE(5, 4) SSSI_CASE(5, 8) SSSI_CASE(6, 4) SSSI_CASE(6, 8)
+            SSSI_CASE(7, 4) SSSI_CASE(7, 8) SSSI_CASE(8, 4) SSSI_CASE(8, 8)
 #undef SSSI_CASE
             default: fail(SSSI_EINVAL, "no kernel for nDiffOrder=%d rz=%d", N, rz);
         }
