@@ -107,16 +107,16 @@ void run_oneshot(Prop &P, int64_t nt, bool reverse, double *data, double *snap, 
             SSSI_HIP(hipMemcpyAsync(host.data(), stage.p, (size_t)ns * cells * sizeof(float),
                                     hipMemcpyDeviceToHost, P.stream));
             SSSI_HIP(hipStreamSynchronize(P.stream));
-            // device slice is x fastest ([k][i][j] for 3-D: ((k*nz)+i)*nx + j);
-            // reference layout is z fastest ((k*nx)+j)*nz + i
+            // device slice is x fastest, [z][y][x]: (i*ny + k)*nx + j;
+            // reference layout is z fastest: (k*nx + j)*nz + i
             const int64_t nz = P.nz, nx = P.nx, ny = P.ny;
             for (int64_t s = 0; s < ns; ++s) {
                 const float *src = host.data() + s * cells;
                 double *dst = snap + (s_lo + s) * cells;
-                for (int64_t k = 0; k < ny; ++k)
-                    for (int64_t i = 0; i < nz; ++i)
+                for (int64_t i = 0; i < nz; ++i)
+                    for (int64_t k = 0; k < ny; ++k)
                         for (int64_t j = 0; j < nx; ++j)
-                            dst[(k * nx + j) * nz + i] = src[(k * nz + i) * nx + j];
+                            dst[(k * nx + j) * nz + i] = src[(i * ny + k) * nx + j];
             }
             done += count;
         }

@@ -1,9 +1,580 @@
-// prop3d.hip -- 3-D CPML propagator (placeholder until the z-sweep kernel lands).
+// prop3d.hip -- 3-D acoustic CPML propagator for gfx950 (fwdTimeCpmlFor3dAw,
+// src/fwdTimeCpmlFor3dAw.m:37-160), including its damping conventions
+// (fwdTimeCpmlFor3dAw.m:46-70: u in grid-index units, L = nBoundary; the x and
+// z profiles take v from the first y slice, the y profile from the top row).
+//
+// Device layout: [z][y][x], x fastest; interior cell (i, k, j) = (z, y, x) at
+// ((i + kHZ) * npy + (k + kHY)) * pitch + (j + kHX).  Halos are zero.
+//
+// One launch per time step.  A block of 64 x TY threads owns a 64 (x) by TY (y)
+// column tile and sweeps a chunk of ZC planes downwards (2.5-D): the z
+// derivative runs from registers (a window of 2L+1 p values and a queue of
+// 2N A_z half-point values per column, so each D+ is computed once), x and y
+// from a double-buffered LDS plane tile with an L-wide halo.  Column tiles
+// whose x/y reach touches no damping strip use the composite D-D+ stencil in
+// x and y; strip tiles run the two-stage CPML form (A_x, A_y at half points
+// through LDS).  The next plane's global loads are issued before the current
+// plane is computed.  Phi is ping-ponged (neighbouring tiles/chunks recompute
+// the half points at their edges), Psi is updated in place.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+
 #include "prop.h"
 
 namespace sssi {
-Prop *make_prop3d(const double *, int64_t, int64_t, int64_t, int, int, double, double, double,
-                  double) {
-    fail(SSSI_ESTATE, "3-D propagator not built");
+
+constexpr int kHY = 16;  // y halo (>= L for N <= 8)
+
+struct Cpml3dArgs {
+    const float *pcur;
+    float *pold;  // read p_old, write p_new
+    const float *vdt2;
+    int nz, nx, ny, pitch, npy, nB;
+    int ntx, nty, zc;  // tile counts in x, y; planes per chunk
+    uint32_t rowb, planeb, nbytes;
+    float cz[8], cx[8], cy[8];
+    float w0, wx[16], wy[16];  // composite x/y weights / d^2 (w0: centre of both)
+    // z strip: half points / rows i in [nz-nB, nz): [(i-(nz-nB))*ny + k]*nx + j
+    const float *zphi_in;
+    float *zphi_out, *zpsi;
+    const float *zbm1;  // [(i-(nz-nB))*nx + j]  (replicated over y)
+    // x strip: [(i*ny + k)*2nB + xs]
+    const float *xphi_in;
+    float *xphi_out, *xpsi;
+    const float *xbm1;  // [i*2nB + xs]           (replicated over y)
+    // y strip: [(i*2nB + ys)*nx + j]
+    const float *yphi_in;
+    float *yphi_out, *ypsi;
+    const float *ybm1;  // [ys*nx + j]            (replicated over z)
+    const int *src_cell;  // (i*ny + k)*nx + j, sorted
+    const float *src_amp;
+    int nsrc;
+    float *rec;   // this step's receivers [k*nx + j] or null
+    float *snap;  // this step's snapshot [(i*ny + k)*nx + j] or null
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc3(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000);
 }
+__device__ __forceinline__ float ld3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void st3(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                    uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+}
+
+template <int N, int TY>
+__global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
+    constexpr int L = 2 * N - 1;
+    constexpr int NT = 64 * TY;
+    constexpr int PX = kTX + 2 * L;  // LDS plane tile: x0-L .. x0+63+L
+    constexpr int PY = TY + 2 * L;   //                 y0-L .. y0+TY-1+L
+    constexpr int HN = PX * PY - kTX * TY;  // halo values per plane
+    constexpr int HPT = (HN + NT - 1) / NT;
+    constexpr int AXW = kTX + L;  // A_x half points x0-N .. x0+63+N-1 per row
+    constexpr int AYH = TY + L;   // A_y half points y0-N .. y0+TY-1+N-1 per column
+    __shared__ float sp[2][PX * PY];
+    __shared__ float sax[TY * AXW];
+    __shared__ float say[AYH * kTX];
+
+    const int tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
+    int b = blockIdx.x;
+    const int tx = b % a.ntx;
+    b /= a.ntx;
+    const int tyi = b % a.nty;
+    const int tz = b / a.nty;
+    const int x0 = tx * kTX, y0 = tyi * TY;
+    const int nz = a.nz, nx = a.nx, ny = a.ny, nB = a.nB;
+    const int zbeg = tz * a.zc, zend = min(zbeg + a.zc, nz);
+    const int j = x0 + lane, k = y0 + ty;
+    const bool own = j < nx && k < ny;
+    const __amdgpu_buffer_rsrc_t rP = buf_rsrc3(a.pcur, a.nbytes);
+    const __amdgpu_buffer_rsrc_t rO = buf_rsrc3(a.pold, a.nbytes);
+    const __amdgpu_buffer_rsrc_t rV = buf_rsrc3(a.vdt2, a.nbytes);
+    const uint32_t vcol = ((uint32_t)(k + kHY) * a.pitch + (uint32_t)(j + kHX)) * 4u;
+    auto plane = [&](int i) -> uint32_t { return (uint32_t)(i + kHZ) * a.planeb; };
+
+    // halo slots of this thread in the LDS plane tile (fixed for the sweep)
+    int h_lds[HPT];
+    uint32_t h_voff[HPT];
+#pragma unroll
+    for (int u = 0; u < HPT; ++u) {
+        const int e = tid + NT * u;
+        int lidx = -1, gy = 0, gx = 0;
+        if (e < HN) {
+            // enumerate tile cells that are not owned: rows of the y halo first
+            const int top = L * PX;                 // y0-L .. y0-1
+            const int mid = TY * 2 * L;             // x halo of the owned rows
+            if (e < top) {
+                const int r = e / PX, c = e - r * PX;
+                lidx = r * PX + c; gy = y0 - L + r; gx = x0 - L + c;
+            } else if (e < top + mid) {
+                const int q = e - top, r = q / (2 * L), c = q - r * (2 * L);
+                const int cc = c < L ? c : kTX + c;
+                lidx = (L + r) * PX + cc; gy = y0 + r; gx = x0 - L + cc;
+            } else {
+                const int q = e - top - mid, r = q / PX, c = q - r * PX;
+                lidx = (L + TY + r) * PX + c; gy = y0 + TY + r; gx = x0 - L + c;
+            }
+        }
+        h_lds[u] = lidx;
+        h_voff[u] = ((uint32_t)(gy + kHY) * a.pitch + (uint32_t)(gx + kHX)) * 4u;
+    }
+
+    const bool xstrip = nB > 0 && (x0 - N < nB || x0 + kTX + N - 1 > nx - nB);
+    const bool ystrip = nB > 0 && (y0 - N < nB || y0 + TY + N - 1 > ny - nB);
+    const int zb0 = nz - nB;
+    const int nb2 = 2 * nB;
+
+    // z window p(i-L .. i+L) and A_z queue (half points i-N .. i+N-1)
+    float zq[2 * L + 1], aq[2 * N];
+#pragma unroll
+    for (int r = 0; r < 2 * L; ++r) zq[r + 1] = ld3(rP, vcol, plane(zbeg - L + r));
+    // A_z at half points zbeg-N .. zbeg+N-2 (aq[1..2N-1])
+#pragma unroll
+    for (int q = 0; q < 2 * N - 1; ++q) {
+        // half point h = zbeg-N+q uses p(h-N+1 .. h+N) = zq[1+q .. 1+q+2N-1]
+        float d = 0.f;
+#pragma unroll
+        for (int m = 0; m < N; ++m) d = fmaf(a.cz[m], zq[1 + q + N + m] - zq[q + N - m], d);
+        const int h = zbeg - N + q;
+        if (nB > 0 && own && h >= zb0 && h < nz) {
+            const size_t o = ((size_t)(h - zb0) * ny + k) * nx + j;
+            const float phi = fmaf(a.zbm1[(size_t)(h - zb0) * nx + j], a.zphi_in[o] + d,
+                                   a.zphi_in[o]);
+            d += phi;
+            if (h >= zbeg && h < zend) a.zphi_out[o] = phi;
+        }
+        aq[q + 1] = d;
+    }
+
+    // prefetch registers for plane zbeg
+    float f_front = ld3(rP, vcol, plane(zbeg + L));
+    float f_po = ld3(rO, vcol, plane(zbeg));
+    float f_v2 = ld3(rV, vcol, plane(zbeg));
+    float f_h[HPT];
+#pragma unroll
+    for (int u = 0; u < HPT; ++u) f_h[u] = h_lds[u] >= 0 ? ld3(rP, h_voff[u], plane(zbeg)) : 0.f;
+
+    int buf = 0;
+    for (int i = zbeg; i < zend; ++i) {
+        // ---- advance the z window to p(i-L .. i+L) and the A_z queue
+#pragma unroll
+        for (int r = 0; r < 2 * L; ++r) zq[r] = zq[r + 1];
+        zq[2 * L] = f_front;
+#pragma unroll
+        for (int q = 0; q < 2 * N - 1; ++q) aq[q] = aq[q + 1];
+        {
+            // new half point h = i+N-1: p(i .. i+2N-1) = zq[L .. L+2N-1]
+            float d = 0.f;
+#pragma unroll
+            for (int m = 0; m < N; ++m) d = fmaf(a.cz[m], zq[L + N + m] - zq[L + N - 1 - m], d);
+            const int h = i + N - 1;
+            if (nB > 0 && own && h >= zb0 && h < nz) {
+                const size_t o = ((size_t)(h - zb0) * ny + k) * nx + j;
+                const float ph = a.zphi_in[o];
+                const float phi = fmaf(a.zbm1[(size_t)(h - zb0) * nx + j], ph + d, ph);
+                d += phi;
+                if (h >= zbeg && h < zend) a.zphi_out[o] = phi;
+            }
+            aq[2 * N - 1] = d;
+        }
+        const float po = f_po, v2 = f_v2;
+        float hv[HPT];
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) hv[u] = f_h[u];
+        // ---- prefetch plane i+1
+        if (i + 1 < zend) {
+            f_front = ld3(rP, vcol, plane(i + 1 + L));
+            f_po = ld3(rO, vcol, plane(i + 1));
+            f_v2 = ld3(rV, vcol, plane(i + 1));
+#pragma unroll
+            for (int u = 0; u < HPT; ++u)
+                f_h[u] = h_lds[u] >= 0 ? ld3(rP, h_voff[u], plane(i + 1)) : 0.f;
+        }
+
+        // ---- P_z = D-A_z + Psi_z
+        float pz = 0.f;
+#pragma unroll
+        for (int m = 0; m < N; ++m) pz = fmaf(a.cz[m], aq[N + m] - aq[N - 1 - m], pz);
+        if (nB > 0 && own && i >= zb0) {
+            const size_t o = ((size_t)(i - zb0) * ny + k) * nx + j;
+            const float ps = a.zpsi[o];
+            const float psi = fmaf(a.zbm1[(size_t)(i - zb0) * nx + j], ps + pz, ps);
+            a.zpsi[o] = psi;
+            pz += psi;
+        }
+
+        // ---- plane i into LDS
+        float *P = sp[buf];
+        P[(ty + L) * PX + lane + L] = zq[L];
+#pragma unroll
+        for (int u = 0; u < HPT; ++u)
+            if (h_lds[u] >= 0) P[h_lds[u]] = hv[u];
+        __syncthreads();
+
+        float pxy;
+        if (!xstrip && !ystrip) {
+            const float *c = P + (ty + L) * PX + lane + L;
+            float acc = a.w0 * c[0];
+#pragma unroll
+            for (int m = 1; m <= L; ++m) acc = fmaf(a.wx[m], c[m] + c[-m], acc);
+#pragma unroll
+            for (int m = 1; m <= L; ++m) acc = fmaf(a.wy[m], c[m * PX] + c[-m * PX], acc);
+            pxy = acc;
+        } else {
+            // A_x at half points hx = x0-N+q (q < 64+L) for row ty
+            const float *row = P + (ty + L) * PX;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int q = lane + half * kTX;
+                if (half == 1 && lane >= L) break;
+                float d = 0.f;
+#pragma unroll
+                for (int m = 0; m < N; ++m) d = fmaf(a.cx[m], row[q + N + m] - row[q + N - 1 - m], d);
+                const int hx = x0 - N + q;
+                if (xstrip && k < ny && hx >= 0 && hx < nx && (hx < nB || hx >= nx - nB)) {
+                    const int xs = hx < nB ? hx : hx - (nx - nb2);
+                    const size_t o = ((size_t)i * ny + k) * nb2 + xs;
+                    const float ph = a.xphi_in[o];
+                    const float phi = fmaf(a.xbm1[(size_t)i * nb2 + xs], ph + d, ph);
+                    d += phi;
+                    if (hx >= x0 && hx < x0 + kTX) a.xphi_out[o] = phi;
+                }
+                sax[ty * AXW + q] = d;
+            }
+            // A_y at half points hy = y0-N+q (q < TY+L) for column lane
+            for (int q = ty; q < AYH; q += TY) {
+                const float *cl = P + lane + L;
+                float d = 0.f;
+#pragma unroll
+                for (int m = 0; m < N; ++m)
+                    d = fmaf(a.cy[m], cl[(q + N + m) * PX] - cl[(q + N - 1 - m) * PX], d);
+                const int hy = y0 - N + q;
+                if (ystrip && j < nx && hy >= 0 && hy < ny && (hy < nB || hy >= ny - nB)) {
+                    const int ys = hy < nB ? hy : hy - (ny - nb2);
+                    const size_t o = ((size_t)i * nb2 + ys) * nx + j;
+                    const float ph = a.yphi_in[o];
+                    const float phi = fmaf(a.ybm1[(size_t)ys * nx + j], ph + d, ph);
+                    d += phi;
+                    if (hy >= y0 && hy < y0 + TY) a.yphi_out[o] = phi;
+                }
+                say[q * kTX + lane] = d;
+            }
+            __syncthreads();
+            float px = 0.f, py = 0.f;
+            const float *ax = sax + ty * AXW + lane + N;
+#pragma unroll
+            for (int m = 0; m < N; ++m) px = fmaf(a.cx[m], ax[m] - ax[-1 - m], px);
+            const float *ay = say + (ty + N) * kTX + lane;
+#pragma unroll
+            for (int m = 0; m < N; ++m) py = fmaf(a.cy[m], ay[m * kTX] - ay[(-1 - m) * kTX], py);
+            if (xstrip && own && (j < nB || j >= nx - nB)) {
+                const int xs = j < nB ? j : j - (nx - nb2);
+                const size_t o = ((size_t)i * ny + k) * nb2 + xs;
+                const float ps = a.xpsi[o];
+                const float psi = fmaf(a.xbm1[(size_t)i * nb2 + xs], ps + px, ps);
+                a.xpsi[o] = psi;
+                px += psi;
+            }
+            if (ystrip && own && (k < nB || k >= ny - nB)) {
+                const int ys = k < nB ? k : k - (ny - nb2);
+                const size_t o = ((size_t)i * nb2 + ys) * nx + j;
+                const float ps = a.ypsi[o];
+                const float psi = fmaf(a.ybm1[(size_t)ys * nx + j], ps + py, ps);
+                a.ypsi[o] = psi;
+                py += psi;
+            }
+            pxy = px + py;
+        }
+
+        // ---- update
+        if (own) {
+            float s = 0.f;
+            if (a.nsrc > 0) {
+                const int cell = (i * ny + k) * nx + j;
+                for (int q = 0; q < a.nsrc; ++q)
+                    if (a.src_cell[q] == cell) s += a.src_amp[q];
+            }
+            const float pc = zq[L];
+            const float pn = pc + fmaf(v2, (pz + pxy) + s, pc - po);
+            st3(pn, rO, vcol, plane(i));
+            if (a.rec && i == 0) a.rec[(size_t)k * nx + j] = pn;
+            if (a.snap) a.snap[((size_t)i * ny + k) * nx + j] = pn;
+        }
+        buf ^= 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+static void composite_weights3(int N, const double *c, double *w) {
+    const int L = 2 * N - 1;
+    for (int m = 0; m <= 2 * L; ++m) w[m] = 0.0;
+    for (int k = 0; k < N; ++k)
+        for (int q = 0; q < N; ++q) {
+            const double cc = c[k] * c[q];
+            w[L + k + 1 + q] += cc;
+            w[L + k - q] -= cc;
+            w[L - k + q] -= cc;
+            w[L - 1 - k - q] += cc;
+        }
+}
+
+class Prop3D final : public Prop {
+   public:
+    int64_t pitch = 0, npy = 0, nzp = 0;
+    DevBuf<float> p[2], vfield;
+    DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1, yphi[2], ypsi, ybm1;
+    int cur = 0, phase = 0;
+    float cz[8] = {0}, cx[8] = {0}, cy[8] = {0}, w0 = 0.f, wx[16] = {0}, wy[16] = {0};
+    int ty = 8, zc = 64;
+
+    Prop3D(const double *v, int64_t nz_, int64_t nx_, int64_t ny_, int N_, int nB_, double dz_,
+           double dx_, double dy_, double dt_) {
+        dim = 3;
+        abc = SSSI_ABC_CPML;
+        nz = nz_; nx = nx_; ny = ny_; N = N_; nB = nB_;
+        dz = dz_; dx = dx_; dy = dy_; dt = dt_;
+        if (nz * nx * ny > ((int64_t)1 << 31) / 2)
+            fail(SSSI_EINVAL, "3-D grid too large (%lld cells)", (long long)(nz * nx * ny));
+        SSSI_HIP(hipGetDevice(&device));
+        SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (const char *e = std::getenv("SSSI_TY3")) {
+            const int t = std::atoi(e);
+            if (t == 4 || t == 8 || t == 16) ty = t;
+        }
+        if (const char *e = std::getenv("SSSI_ZC3")) {
+            const int z = std::atoi(e);
+            if (z >= 8 && z <= 4096) zc = z;
+        }
+        pitch = round_up(kHX + round_up(nx, kTX) + 32, 64);
+        npy = kHY + round_up(ny, 16) + kHY;
+        nzp = kHZ + nz + kHZbot;
+        const size_t np = (size_t)pitch * npy * nzp;
+        if (np * 4 >= ((size_t)1 << 32)) fail(SSSI_EINVAL, "3-D padded grid exceeds 4 GiB");
+        p[0].alloc(np);
+        p[1].alloc(np);
+        vfield.alloc(np);
+        {
+            std::vector<float> hv(np, 0.f);
+            for (int64_t k = 0; k < ny; ++k)
+                for (int64_t j = 0; j < nx; ++j)
+                    for (int64_t i = 0; i < nz; ++i) {
+                        const double vv = v[(k * nx + j) * nz + i];
+                        hv[((size_t)(i + kHZ) * npy + k + kHY) * pitch + j + kHX] =
+                            (float)((vv * dt) * (vv * dt));
+                    }
+            SSSI_HIP(hipMemcpy(vfield.p, hv.data(), np * sizeof(float), hipMemcpyHostToDevice));
+        }
+        double c[8];
+        staggered_coefficients(N, c);
+        for (int q = 0; q < N; ++q) {
+            cz[q] = (float)(c[q] / dz);
+            cx[q] = (float)(c[q] / dx);
+            cy[q] = (float)(c[q] / dy);
+        }
+        double w[32];
+        composite_weights3(N, c, w);
+        const int L = 2 * N - 1;
+        w0 = (float)(w[L] / (dx * dx) + w[L] / (dy * dy));
+        for (int m = 1; m <= L; ++m) {
+            wx[m] = (float)(w[L + m] / (dx * dx));
+            wy[m] = (float)(w[L + m] / (dy * dy));
+        }
+        if (nB > 0) {
+            // fwdTimeCpmlFor3dAw.m:46-70 with dampPml.m: u in samples, L = nB
+            const double logR = std::log(1e-6), Lb = (double)nB;
+            auto bm1 = [&](double vv, double u) {
+                const double d0 = -(3 * vv) / (2 * Lb) * logR;
+                const double q = (u / Lb) * (u / Lb);
+                return (float)(std::exp(-(d0 * q) * dt) - 1.0);
+            };
+            auto V = [&](int64_t i, int64_t j, int64_t k) { return v[(k * nx + j) * nz + i]; };
+            std::vector<float> hx((size_t)nz * 2 * nB), hy((size_t)2 * nB * nx), hz((size_t)nB * nx);
+            for (int64_t i = 0; i < nz; ++i)
+                for (int q = 0; q < nB; ++q) {
+                    hx[(size_t)i * 2 * nB + q] = bm1(V(i, q, 0), (double)(nB - q));
+                    hx[(size_t)i * 2 * nB + nB + q] = bm1(V(i, nx - nB + q, 0), (double)(q + 1));
+                }
+            for (int q = 0; q < nB; ++q)
+                for (int64_t j = 0; j < nx; ++j) {
+                    hy[(size_t)q * nx + j] = bm1(V(0, j, q), (double)(nB - q));
+                    hy[(size_t)(nB + q) * nx + j] = bm1(V(0, j, ny - nB + q), (double)(q + 1));
+                }
+            for (int q = 0; q < nB; ++q)
+                for (int64_t j = 0; j < nx; ++j)
+                    hz[(size_t)q * nx + j] = bm1(V(nz - nB + q, j, 0), (double)(q + 1));
+            xbm1.alloc(hx.size());
+            ybm1.alloc(hy.size());
+            zbm1.alloc(hz.size());
+            SSSI_HIP(hipMemcpy(xbm1.p, hx.data(), xbm1.bytes(), hipMemcpyHostToDevice));
+            SSSI_HIP(hipMemcpy(ybm1.p, hy.data(), ybm1.bytes(), hipMemcpyHostToDevice));
+            SSSI_HIP(hipMemcpy(zbm1.p, hz.data(), zbm1.bytes(), hipMemcpyHostToDevice));
+            const size_t zn = (size_t)nB * ny * nx, xn = (size_t)nz * ny * 2 * nB,
+                         yn = (size_t)nz * 2 * nB * nx;
+            for (int q = 0; q < 2; ++q) {
+                zphi[q].alloc(zn);
+                xphi[q].alloc(xn);
+                yphi[q].alloc(yn);
+            }
+            zpsi.alloc(zn);
+            xpsi.alloc(xn);
+            ypsi.alloc(yn);
+        }
+        reset();
+    }
+
+    ~Prop3D() override {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int64_t cells() const override { return nz * nx * ny; }
+
+    void reset() override {
+        for (auto *b : {&p[0], &p[1], &zphi[0], &zphi[1], &zpsi, &xphi[0], &xphi[1], &xpsi,
+                        &yphi[0], &yphi[1], &ypsi})
+            if (b->p) SSSI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), stream));
+        cur = 0;
+        phase = 0;
+        SSSI_HIP(hipStreamSynchronize(stream));
+    }
+
+    void set_points(int n, const int *iz, const int *ix, const int *iy, const double *wavelet,
+                    int64_t nt) override {
+        if (n < 0 || nt < 1 || (n > 0 && (!iz || !ix || !iy || !wavelet)))
+            fail(SSSI_EINVAL, "set_points: bad arguments");
+        if (n > 4096) fail(SSSI_EINVAL, "set_points: at most 4096 point sources in 3-D");
+        for (int q = 0; q < n; ++q)
+            if (iz[q] < 0 || iz[q] >= nz || ix[q] < 0 || ix[q] >= nx || iy[q] < 0 || iy[q] >= ny)
+                fail(SSSI_EINVAL, "set_points: source %d outside the grid", q);
+        nsrc = n;
+        src_nt = nt;
+        if (n == 0) {
+            src_cell.release();
+            src_amp.release();
+            return;
+        }
+        std::vector<int> cell(n);
+        for (int q = 0; q < n; ++q) cell[q] = (int)(((int64_t)iz[q] * ny + iy[q]) * nx + ix[q]);
+        std::vector<float> amp((size_t)nt * n);
+        for (size_t q = 0; q < amp.size(); ++q) amp[q] = (float)wavelet[q];
+        src_cell.alloc(n);
+        src_amp.alloc(amp.size());
+        SSSI_HIP(hipMemcpy(src_cell.p, cell.data(), src_cell.bytes(), hipMemcpyHostToDevice));
+        SSSI_HIP(hipMemcpy(src_amp.p, amp.data(), src_amp.bytes(), hipMemcpyHostToDevice));
+    }
+
+    void set_surface(const double *, int64_t) override {
+        fail(SSSI_ESTATE, "3-D handles run forward modelling only");
+    }
+
+    template <int NN, int TY>
+    void launch(const Cpml3dArgs &args, int nblk) {
+        hipLaunchKernelGGL((cpml3d_step<NN, TY>), dim3(nblk), dim3(64 * TY), 0, stream, args);
+    }
+
+    void step(int64_t t, bool record, float *snap) {
+        Cpml3dArgs a{};
+        a.pcur = p[cur].p;
+        a.pold = p[cur ^ 1].p;
+        a.vdt2 = vfield.p;
+        a.nz = (int)nz; a.nx = (int)nx; a.ny = (int)ny;
+        a.pitch = (int)pitch; a.npy = (int)npy; a.nB = nB;
+        a.ntx = (int)((nx + kTX - 1) / kTX);
+        a.nty = (int)((ny + ty - 1) / ty);
+        a.zc = zc;
+        a.rowb = (uint32_t)pitch * 4u;
+        a.planeb = (uint32_t)(pitch * npy) * 4u;
+        a.nbytes = (uint32_t)(p[0].bytes());
+        for (int q = 0; q < 8; ++q) { a.cz[q] = cz[q]; a.cx[q] = cx[q]; a.cy[q] = cy[q]; }
+        a.w0 = w0;
+        for (int m = 0; m < 16; ++m) { a.wx[m] = wx[m]; a.wy[m] = wy[m]; }
+        if (nB > 0) {
+            a.zphi_in = zphi[phase].p; a.zphi_out = zphi[phase ^ 1].p; a.zpsi = zpsi.p; a.zbm1 = zbm1.p;
+            a.xphi_in = xphi[phase].p; a.xphi_out = xphi[phase ^ 1].p; a.xpsi = xpsi.p; a.xbm1 = xbm1.p;
+            a.yphi_in = yphi[phase].p; a.yphi_out = yphi[phase ^ 1].p; a.ypsi = ypsi.p; a.ybm1 = ybm1.p;
+        }
+        if (nsrc > 0 && t < src_nt) {
+            a.src_cell = src_cell.p;
+            a.src_amp = src_amp.p + (size_t)t * nsrc;
+            a.nsrc = nsrc;
+        }
+        if (record) a.rec = rec.p + (size_t)t * nx * ny;
+        a.snap = snap;
+        const int nzc = (int)((nz + zc - 1) / zc);
+        const int nblk = a.ntx * a.nty * nzc;
+        switch (N * 100 + ty) {
+#define SSSI_C3(NN, T) case NN * 100 + T: launch<NN, T>(a, nblk); break;
+            SSSI_C3(1, 4) SSSI_C3(1, 8) SSSI_C3(1, 16) SSSI_C3(2, 4) SSSI_C3(2, 8) SSSI_C3(2, 16)
+            SSSI_C3(3, 4) SSSI_C3(3, 8) SSSI_C3(3, 16) SSSI_C3(4, 4) SSSI_C3(4, 8) SSSI_C3(4, 16)
+            SSSI_C3(5, 4) SSSI_C3(5, 8) SSSI_C3(5, 16) SSSI_C3(6, 4) SSSI_C3(6, 8) SSSI_C3(6, 16)
+            SSSI_C3(7, 4) SSSI_C3(7, 8) SSSI_C3(7, 16) SSSI_C3(8, 4) SSSI_C3(8, 8) SSSI_C3(8, 16)
+#undef SSSI_C3
+            default: fail(SSSI_EINVAL, "no 3-D kernel for nDiffOrder=%d ty=%d", N, ty);
+        }
+        SSSI_HIP(hipGetLastError());
+        cur ^= 1;
+        phase ^= 1;
+    }
+
+    void run(int64_t t_first, int64_t count, bool reverse, bool record, const SnapSink *sink,
+             LaunchStats *st) override {
+        if (reverse) fail(SSSI_ESTATE, "3-D handles run forward modelling only");
+        if (record && rec_nt < t_first + count) {
+            rec.alloc((size_t)(t_first + count) * nx * ny);
+            rec_nt = t_first + count;
+        }
+        for (int64_t q = 0; q < count; ++q) {
+            const int64_t t = t_first + q;
+            float *snap = nullptr;
+            if (sink && sink->dev && t % sink->stride == 0)
+                snap = sink->dev + (size_t)(t / sink->stride - sink->slot0) * cells();
+            step(t, record, snap);
+            if (st) st->launches++;
+        }
+    }
+
+    void get_data(double *data, int64_t nt) override {
+        if (nt > rec_nt || !rec.p) fail(SSSI_ESTATE, "get_data: no recorded traces");
+        std::vector<float> h((size_t)nt * nx * ny);
+        SSSI_HIP(hipMemcpyAsync(h.data(), rec.p, h.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                stream));
+        SSSI_HIP(hipStreamSynchronize(stream));
+        for (size_t q = 0; q < h.size(); ++q) data[q] = h[q];  // [t][k][j] == data(nx,ny,nt)
+    }
+
+    void get_field(double *field) override {
+        const size_t np = (size_t)pitch * npy * nzp;
+        std::vector<float> h(np);
+        SSSI_HIP(hipMemcpyAsync(h.data(), p[cur].p, np * sizeof(float), hipMemcpyDeviceToHost,
+                                stream));
+        SSSI_HIP(hipStreamSynchronize(stream));
+        for (int64_t k = 0; k < ny; ++k)
+            for (int64_t j = 0; j < nx; ++j)
+                for (int64_t i = 0; i < nz; ++i)
+                    field[(k * nx + j) * nz + i] =
+                        h[((size_t)(i + kHZ) * npy + k + kHY) * pitch + j + kHX];
+    }
+
+    void get_model(double *) override { fail(SSSI_ESTATE, "3-D handles have no model output"); }
+
+    void alg_bytes(double *bytes, double *ncells) override {
+        const double c = (double)nz * nx * ny;
+        const double strips = nB > 0 ? (double)nB * nx * ny + (double)nz * ny * 2 * nB +
+                                           (double)nz * nx * 2 * nB
+                                     : 0.0;
+        *bytes = 16.0 * c + 20.0 * strips + 4.0 * nx * ny;
+        *ncells = c;
+    }
+};
+
+Prop *make_prop3d(const double *v, int64_t nz, int64_t nx, int64_t ny, int N, int nB, double dz,
+                  double dx, double dy, double dt) {
+    return new Prop3D(v, nz, nx, ny, N, nB, dz, dx, dy, dt);
+}
+
 }  // namespace sssi
