@@ -171,6 +171,52 @@ int sssi_prop_get_model(sssi_prop *p, double *model);
  * accounting: algorithmic bytes per step of the propagation kernel. */
 int sssi_prop_alg_bytes_per_step(sssi_prop *p, double *bytes, double *cells);
 
+/* ---------------------------------------------------------------------
+ * Reverse-time migration, 2-D CPML (the imaging loop of
+ * applications/rtm_2d_acoustic/mainRtmTimeCpmlFor2dAw.m:147-315).
+ * The model V and the migration (smooth) model VS are physical grids
+ * (nz,nx) column-major.  Each shot works on the window [x_begin,
+ * x_begin+x_count) extended by nBoundary cells (src/extBoundary.m): left,
+ * right and below.  Per shot:
+ *   dataTrue  = fwd(V, src)             (computed once and cached, :165)
+ *   dataSmooth= fwd(VS, src), checkpoints of the source wavefield every
+ *               img_stride steps (FP32, or FP16/BF16 with a per-checkpoint
+ *               scale)
+ *   rvs(V, dataTrue - dataSmooth)       (:252, :256) with the imaging
+ *   condition M += f.*r, s2 += f.^2 at the checkpoint steps (:270-273)
+ *   Stacked(window) += M ./ (eps + s2)  (:315), FP64, in shot order.
+ * img_stride = 1 images every step as the reference does.
+ * ------------------------------------------------------------------- */
+#define SSSI_CK_F32 1
+#define SSSI_CK_F16 2
+#define SSSI_CK_BF16 3
+
+typedef struct sssi_rtm sssi_rtm;
+
+int sssi_rtm2d_create(const double *V, const double *VS, int64_t nz, int64_t nx, int nDiffOrder,
+                      int nBoundary, double dz, double dx, double dt, int64_t nt, int img_stride,
+                      int ck_format, double eps, sssi_rtm **out);
+/* Source at depth 0, column src_ix of the window (physical, 0-based);
+ * wavelet has nt samples.  Returns the shot index in *shot_id. */
+int sssi_rtm2d_add_shot(sssi_rtm *r, int64_t x_begin, int64_t x_count, int src_ix,
+                        const double *wavelet, int *shot_id);
+/* Compute and cache dataTrue for the shots that lack it. */
+int sssi_rtm2d_precompute_data(sssi_rtm *r, int batched);
+/* Migrate every shot and stack.  batched = 0: one shot per launch;
+ * batched = 1: one launch per time step covers all shots.  Returns the
+ * elapsed milliseconds (HIP events, excluding the cached dataTrue) and the
+ * number of kernel launches. */
+int sssi_rtm2d_migrate(sssi_rtm *r, int batched, double *ms, int64_t *launches);
+/* Stacked image (nz,nx) column-major. */
+int sssi_rtm2d_get_stack(sssi_rtm *r, double *stack);
+/* Per-shot results on the extended window (nz+nB, x_count+2nB) column-major
+ * (image, M, s2 without eps) and dataTrue (nx_ext, nt); any may be NULL. */
+int sssi_rtm2d_get_shot(sssi_rtm *r, int shot, double *image, double *num, double *den,
+                        double *data_true);
+/* Device memory held by the handle. */
+int sssi_rtm2d_device_bytes(sssi_rtm *r, double *bytes);
+void sssi_rtm2d_destroy(sssi_rtm *r);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,10 +23,11 @@ from ._lib import (SssiError, lib, device_count, fwd_cpml_2d_points, rvs_cpml_2d
 from .api import (fwdTimeCpmlFor2dAw, rvsTimeCpmlFor2dAw, fwdTimeCpmlFor3dAw,
                   fwdTimeSpongeFor2dAw, rvsTimeSpongeFor2dAw)
 from . import synth
+from .rtm import Rtm2D
 
 __all__ = [
     "SssiError", "lib", "device_count", "fwdTimeCpmlFor2dAw", "rvsTimeCpmlFor2dAw",
     "fwdTimeCpmlFor3dAw", "fwdTimeSpongeFor2dAw", "rvsTimeSpongeFor2dAw",
     "fwd_cpml_2d_points", "rvs_cpml_2d_ex", "fwd_cpml_3d_points", "fwd_sponge_2d_points",
-    "rvs_sponge_2d_ex", "Propagator", "synth",
+    "rvs_sponge_2d_ex", "Propagator", "synth", "Rtm2D",
 ]

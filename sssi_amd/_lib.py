@@ -53,8 +53,18 @@ SIGNATURES = {
     "sssi_prop_get_field": [_vp, _D],
     "sssi_prop_get_model": [_vp, _D],
     "sssi_prop_alg_bytes_per_step": [_vp, _D, _D],
+    "sssi_rtm2d_create": [_D, _D, _i64, _i64, _int, _int, _dbl, _dbl, _dbl, _i64, _int, _int, _dbl,
+                          ctypes.POINTER(_vp)],
+    "sssi_rtm2d_add_shot": [_vp, _i64, _i64, _int, _D, _I],
+    "sssi_rtm2d_precompute_data": [_vp, _int],
+    "sssi_rtm2d_migrate": [_vp, _int, _D, ctypes.POINTER(_i64)],
+    "sssi_rtm2d_get_stack": [_vp, _D],
+    "sssi_rtm2d_get_shot": [_vp, _int, _D, _D, _D, _D],
+    "sssi_rtm2d_device_bytes": [_vp, _D],
+    "sssi_rtm2d_destroy": [_vp],
 }
-_RESTYPE = {"sssi_last_error": ctypes.c_char_p, "sssi_prop_destroy": None}
+_RESTYPE = {"sssi_last_error": ctypes.c_char_p, "sssi_prop_destroy": None,
+            "sssi_rtm2d_destroy": None}
 
 _lib = None
 

@@ -1,0 +1,107 @@
+// cpml2d.h -- 2-D CPML propagation jobs: device descriptor, per-model device
+// state and launch helpers shared by the single-model handles (prop2d.hip) and
+// the batched multi-shot RTM driver (rtm.hip).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "prop.h"
+
+namespace sssi {
+
+// Half-precision checkpoint formats.
+enum CkFmt : int { kCkNone = 0, kCkF32 = 1, kCkF16 = 2, kCkBF16 = 3 };
+constexpr int kCkSlots = 64;  // per-checkpoint |max| slots (spread the atomics)
+
+// Everything a time step of one 2-D CPML propagation needs, resident in
+// device memory (batched launches) or passed by value (single launches).
+// Per-step quantities (time index, ping-pong parity) are derived in-kernel
+// from the launch's step counter, so the descriptor stays fixed for a phase.
+struct Job2D {
+    int nz, nx, pitch, rows, ntx, nwz, nB;
+    int start_step, nt, reverse;  // t = start..: forward t = rel, reverse t = nt-1-rel
+    float cz[8], cx[8];           // c_k/dz, c_k/dx (two-stage form)
+    float w0, wz[16], wx[16];     // composite D-D+ weights / d^2
+    float *p[2];                  // wavefield levels (padded); parity picks current
+    const float *vdt2;            // v^2 dt^2 (padded)
+    float *zphi[2], *zpsi;        // z strip [(i-(nz-nB))*nx + j]
+    const float *zbm1;
+    float *xphi[2], *xpsi;        // x strips [i*2nB + xs]
+    const float *xbm1;
+    // forward sources: CSR over wave tiles (64 x rz), amplitudes [t][k]
+    const int *src_ptr, *src_cell;
+    const float *src_amp;
+    int nsrc, src_nt;
+    // reverse surface injection: s(0,j) = surf[t*nx+j] - surf2[t*nx+j] (surf2 may be null)
+    const float *surf, *surf2;
+    int surf_nt;
+    float *rec;                   // receivers [t*nx + j] or null
+    float *snap;                  // FP32 snapshots at t % snap_stride == 0, slot t/stride - slot0
+    int snap_stride, snap_slot0;
+    // checkpoints of the forward field at t % ck_stride == 0 (slot c = t/ck_stride)
+    int ck_stride, ck_fmt;
+    float *ck32;                  // kCkF32: [c][nz*nx]
+    float *ck_stage;              // kCkF16/kCkBF16: FP32 staging [c&1][nz*nx]
+    uint16_t *ck16;               // [c][nz*nx], value = half(p / scale[c])
+    unsigned *ck_max;             // [c][kCkSlots] bits of max |p| (per wave, atomicMax)
+    float *ck_scale;              // [c]
+    // imaging against the checkpoints (reverse): num += f*p, den += f*f
+    const float *im_ck32;
+    const uint16_t *im_ck16;
+    const float *im_scale;
+    int im_stride, im_fmt;
+    float *im_num, *im_den;
+};
+
+// Per-model device state of a 2-D CPML propagation (velocity-dependent
+// coefficients and damping, wavefield levels, memory variables).
+class Cpml2dGrid {
+   public:
+    int64_t nz = 0, nx = 0, pitch = 0, rows = 0;
+    int N = 1, nB = 0, rz = 4;
+    double dz = 0, dx = 0, dt = 0;
+    DevBuf<float> p[2], vdt2;
+    DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1;
+    float cz[8] = {0}, cx[8] = {0}, w0 = 0.f, wz[16] = {0}, wx[16] = {0};
+
+    Cpml2dGrid(const double *v, int64_t nz, int64_t nx, int N, int nB, double dz, double dx,
+               double dt, int rz = 0);
+    int ntx() const { return (int)((nx + kTX - 1) / kTX); }
+    int nwz() const { return (int)((nz + rz - 1) / rz); }
+    int tz() const { return 4 * rz; }
+    int ntz() const { return (int)((nz + tz() - 1) / tz()); }
+    int nblocks() const { return ntx() * ntz(); }
+    bool block_is_strip(int tile) const;
+    void zero_state(hipStream_t s);
+    // descriptor with geometry, coefficients and state pointers filled in
+    Job2D job() const;
+    // interior (nz x nx, x fastest) of level `which` -> column-major doubles
+    void download(int which, double *out, hipStream_t s) const;
+    double alg_bytes_per_step() const;
+};
+
+// Point sources sorted into the per-wave-tile CSR the kernel scans.
+struct DevSources {
+    DevBuf<int> ptr, cell;
+    DevBuf<float> amp;
+    int n = 0, nt = 0;
+    void set(const Cpml2dGrid &g, int n, const int *iz, const int *ix, const double *wavelet,
+             int64_t nt);
+};
+
+// Block launch order for a set of jobs: (job, block tile) pairs with the
+// strip tiles of every job first, interior tiles split into 8 contiguous bands
+// (block b is dispatched to XCD b % 8).
+void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector<int2> &order);
+
+// Launch one step of a single job (descriptor by value).
+void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
+                          int step, hipStream_t s);
+// Launch one step of many jobs sharing nDiffOrder and rz (descriptors in device memory).
+void launch_cpml2d_batch(int N, int rz, const Job2D *jobs, const int2 *order, int nblk, int step,
+                         hipStream_t s);
+// Pack the staged FP32 checkpoint `c` of a job into 16-bit (after its last step).
+void launch_ck_pack(const Job2D &job, int c, hipStream_t s);
+
+}  // namespace sssi
