@@ -70,17 +70,160 @@ def cpu_baseline(wl, steps=60):
                       f"first {steps} of {wl['nt']} time steps, {threads} OpenMP threads"}
 
 
+def workload_3d():
+    from sssi_amd import synth
+    n0, nB, N = 400, 20, 4
+    d = 10.0
+    dt = 1e-3
+    nt = 2000
+    v = synth.cake_3d_dipping(n0, n0, n0, 8, 1500.0, 4500.0, seed=4)
+    # the 8th-order 3-D scheme at dt = 1 ms, d = 10 m is stable below 4490 m/s
+    v = np.clip(v, 1500.0, 4450.0)
+    V = synth.extBoundary(v, nB, 3)
+    w = synth.ricker(15.0, nt, dt)
+    c = (n0 + 2 * nB) // 2
+    return dict(V=V, nB=nB, N=N, d=d, dt=dt, nt=nt, wavelet=w, src=(0, c, c))
+
+
+def bench_3d(args):
+    """3-D CPML forward, 420 x 440 x 440 ext grid, nDiffOrder = 4, nt = 2000."""
+    import sssi_amd
+    wl = workload_3d()
+    V = wl["V"]
+    nt = wl["nt"] if args.nt is None else args.nt
+    P = sssi_amd.Propagator(V, wl["N"], wl["nB"], wl["d"], wl["d"], wl["dt"], dy=wl["d"])
+    iz, ix, iy = wl["src"]
+    P.set_points([iz], [ix], wl["wavelet"], src_iy=[iy])
+    bytes_step, cells = P.alg_bytes_per_step()
+    for _ in range(args.warmup):
+        P.reset()
+        P.run(min(nt, 50))
+    P.sync()
+    kern_ms, launches = 0.0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        P.reset()
+        ms, n = P.run_timed(nt)
+        kern_ms += ms
+        launches += n
+    wall = time.perf_counter() - t0
+    avg = kern_ms / 1e3 / launches
+    ach = bytes_step / avg / 1e9
+    return {
+        "metric": "Gcell-updates/s (3D acoustic CPML forward, FP32)",
+        "value": round(cells * nt * args.steps / wall / 1e9, 3), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded 8-layer cake with a dipping interface, Ricker source)",
+        "config": {"workload": f"3D CPML forward, {V.shape[0]}x{V.shape[1]}x{V.shape[2]} ext "
+                               f"grid, nDiffOrder=4, nt={nt}, dt=1ms, 1 shot",
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "cpml3d_step", "alg_bytes_per_launch": bytes_step,
+                     "avg_launch_us": round(avg * 1e6, 3), "launches": launches},
+    }
+
+
+def workload_rtm(nshots=64, seed=5):
+    from sssi_amd import synth
+    nz, nx, nB, N = 600, 6000, 30, 4
+    d, dt, nt = 10.0, 1e-3, 3000
+    V = synth.layered_2d(nz, nx, 12, 1500.0, 4500.0, seed=seed)
+    VS = synth.smooth_copy(V, 10)
+    rng = np.random.default_rng(seed + 1)
+    w = synth.ricker(15.0, nt, dt)
+    shots = []
+    for _ in range(nshots):
+        xc = int(rng.integers(400, 1201))
+        xb = int(rng.integers(0, nx - xc + 1))
+        shots.append((xb, xc, xc // 2))
+    return dict(V=V, VS=VS, nB=nB, N=N, d=d, dt=dt, nt=nt, wavelet=w, shots=shots)
+
+
+def bench_rtm(args):
+    """Config-3 RTM: 64 shots of uneven aperture; plain (one shot per launch, FP32
+    checkpoints) vs batched (all shots per launch, 16-bit scaled checkpoints)."""
+    import sssi_amd
+    wl = workload_rtm(args.shots)
+    res = {}
+    for name, batched, fmt in (("plain_fp32", False, "fp32"), ("batched_" + args.ck, True, args.ck)):
+        r = sssi_amd.Rtm2D(wl["V"], wl["VS"], wl["N"], wl["nB"], wl["d"], wl["d"], wl["dt"],
+                           wl["nt"], img_stride=5, ck_format=fmt)
+        for xb, xc, s in wl["shots"]:
+            r.add_shot(xb, xc, s, wl["wavelet"])
+        r.precompute_data(batched=True)
+        t0 = time.perf_counter()
+        ms, launches = r.migrate(batched=batched)
+        wall = time.perf_counter() - t0
+        res[name] = {"shots_per_hour": round(len(wl["shots"]) / wall * 3600, 1),
+                     "seconds": round(wall, 3), "event_ms": round(ms, 2), "launches": launches,
+                     "device_GB": round(r.device_bytes() / 1e9, 2)}
+        r.close()
+    b = res["batched_" + args.ck]["shots_per_hour"]
+    p = res["plain_fp32"]["shots_per_hour"]
+    return {
+        "metric": "RTM shots/hour (2D CPML, 64 shots, batched + 16-bit checkpoints)",
+        "value": b, "unit": "shots/hour", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded 600x6000 layered model, 10-cell Gaussian-blurred copy)",
+        "config": {"workload": "2D RTM, 64 shots, nz=600, nx~U[400,1200], nB=30, "
+                               "nDiffOrder=4, nt=3000, imaging every 5 steps",
+                   "parallelism": "1 GPU"},
+        "speedup_vs_plain_fp32": round(b / p, 3), "modes": res,
+    }
+
+
+def bench_copy(args):
+    """Device-to-device copy of a known size (hipMemcpy's blit kernel), the
+    calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 0, ctypes.c_size_t(n))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"metric": "device copy GB/s (calibration)", "value": round(2 * n * args.steps / wall / 1e9, 1),
+            "unit": "GB/s", "bytes_read_per_copy": n, "bytes_written_per_copy": n,
+            "steps": args.steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "rtm", "copy"],
+                    help="2d: the headline 2-D forward line (default); 3d / rtm: other workloads")
+    ap.add_argument("--nt", type=int, default=None, help="3d: time steps (default 2000)")
+    ap.add_argument("--shots", type=int, default=64, help="rtm: number of shots")
+    ap.add_argument("--ck", default="bf16", choices=["fp16", "bf16"], help="rtm: checkpoint type")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are "
               "not part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "3d":
+        print(json.dumps(bench_3d(args)), flush=True)
+        return
+    if args.workload == "rtm":
+        print(json.dumps(bench_rtm(args)), flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
+        return
 
     import sssi_amd
     wl = workload()

@@ -22,7 +22,7 @@ struct Job2D {
     int nz, nx, pitch, rows, ntx, nwz, nB;
     int start_step, nt, reverse;  // t = start..: forward t = rel, reverse t = nt-1-rel
     float cz[8], cx[8];           // c_k/dz, c_k/dx (two-stage form)
-    float w0, wz[16], wx[16];     // composite D-D+ weights / d^2
+    float w0, w0lo, wz[16], wx[16];  // composite D-D+ weights / d^2 (w0 + w0lo: centre)
     float *p[2];                  // wavefield levels (padded); parity picks current
     const float *vdt2;            // v^2 dt^2 (padded)
     float *zphi[2], *zpsi;        // z strip [(i-(nz-nB))*nx + j]
@@ -63,7 +63,7 @@ class Cpml2dGrid {
     double dz = 0, dx = 0, dt = 0;
     DevBuf<float> p[2], vdt2;
     DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1;
-    float cz[8] = {0}, cx[8] = {0}, w0 = 0.f, wz[16] = {0}, wx[16] = {0};
+    float cz[8] = {0}, cx[8] = {0}, w0 = 0.f, w0lo = 0.f, wz[16] = {0}, wx[16] = {0};
 
     Cpml2dGrid(const double *v, int64_t nz, int64_t nx, int N, int nB, double dz, double dx,
                double dt, int rz = 0);

@@ -135,7 +135,9 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step) 
             for (int m = 1; m <= L; ++m) acc = fmaf(J.wz[m], col[L + r + m] + col[L + r - m], acc);
 #pragma unroll
             for (int m = 1; m <= L; ++m) acc = fmaf(J.wx[m], row[m] + row[-m], acc);
-            lap[r] = acc;
+            // the low part of the centre weight, added once the large terms have
+            // cancelled, makes the weights sum to zero to FP32 resolution of lap
+            lap[r] = fmaf(J.w0lo, col[L + r], acc);
         }
     } else {
         // ---- damping strips: two-stage form with Phi/Psi
@@ -427,11 +429,17 @@ Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB
     double w[32];
     composite_weights(N, c, w);
     const int L = 2 * N - 1;
-    w0 = (float)(w[L] / (dz * dz) + w[L] / (dx * dx));
+    double side = 0.0;  // exact in double: a sum of 4L floats
     for (int m = 1; m <= L; ++m) {
         wz[m] = (float)(w[L + m] / (dz * dz));
         wx[m] = (float)(w[L + m] / (dx * dx));
+        side += 2.0 * ((double)wz[m] + (double)wx[m]);
     }
+    // The centre weight is chosen so that the FP32 weights sum to zero (the
+    // Laplacian of a constant vanishes, as in the two-stage form where
+    // D+ of a constant is exactly 0); it is carried as w0 + w0lo.
+    w0 = (float)(-side);
+    w0lo = (float)(-side - (double)w0);
     if (nB > 0) {
         std::vector<float> zb, xb;
         cpml2d_bm1(v, nz, nx, nB, dz, dx, dt, zb, xb);
@@ -469,6 +477,7 @@ Job2D Cpml2dGrid::job() const {
     J.ntx = ntx(); J.nwz = nwz(); J.nB = nB;
     for (int k = 0; k < 8; ++k) { J.cz[k] = cz[k]; J.cx[k] = cx[k]; }
     J.w0 = w0;
+    J.w0lo = w0lo;
     for (int m = 0; m < 16; ++m) { J.wz[m] = wz[m]; J.wx[m] = wx[m]; }
     J.p[0] = p[0].p; J.p[1] = p[1].p;
     J.vdt2 = vdt2.p;

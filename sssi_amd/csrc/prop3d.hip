@@ -35,7 +35,7 @@ struct Cpml3dArgs {
     int ntx, nty, zc;  // tile counts in x, y; planes per chunk
     uint32_t rowb, planeb, nbytes;
     float cz[8], cx[8], cy[8];
-    float w0, wx[16], wy[16];  // composite x/y weights / d^2 (w0: centre of both)
+    float w0, w0lo, wx[16], wy[16];  // composite x/y weights / d^2 (w0 + w0lo: centre)
     // z strip: half points / rows i in [nz-nB, nz): [(i-(nz-nB))*ny + k]*nx + j
     const float *zphi_in;
     float *zphi_out, *zpsi;
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
             for (int m = 1; m <= L; ++m) acc = fmaf(a.wx[m], c[m] + c[-m], acc);
 #pragma unroll
             for (int m = 1; m <= L; ++m) acc = fmaf(a.wy[m], c[m * PX] + c[-m * PX], acc);
-            pxy = acc;
+            pxy = fmaf(a.w0lo, c[0], acc);  // low part of the centre weight, after cancellation
         } else {
             // A_x at half points hx = x0-N+q (q < 64+L) for row ty
             const float *row = P + (ty + L) * PX;
@@ -330,7 +330,7 @@ class Prop3D final : public Prop {
     DevBuf<float> p[2], vfield;
     DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1, yphi[2], ypsi, ybm1;
     int cur = 0, phase = 0;
-    float cz[8] = {0}, cx[8] = {0}, cy[8] = {0}, w0 = 0.f, wx[16] = {0}, wy[16] = {0};
+    float cz[8] = {0}, cx[8] = {0}, cy[8] = {0}, w0 = 0.f, w0lo = 0.f, wx[16] = {0}, wy[16] = {0};
     int ty = 8, zc = 64;
 
     Prop3D(const double *v, int64_t nz_, int64_t nx_, int64_t ny_, int N_, int nB_, double dz_,
@@ -380,11 +380,15 @@ class Prop3D final : public Prop {
         double w[32];
         composite_weights3(N, c, w);
         const int L = 2 * N - 1;
-        w0 = (float)(w[L] / (dx * dx) + w[L] / (dy * dy));
+        double side = 0.0;  // exact in double
         for (int m = 1; m <= L; ++m) {
             wx[m] = (float)(w[L + m] / (dx * dx));
             wy[m] = (float)(w[L + m] / (dy * dy));
+            side += 2.0 * ((double)wx[m] + (double)wy[m]);
         }
+        // centre weight makes the FP32 weights sum to zero, carried as w0 + w0lo
+        w0 = (float)(-side);
+        w0lo = (float)(-side - (double)w0);
         if (nB > 0) {
             // fwdTimeCpmlFor3dAw.m:46-70 with dampPml.m: u in samples, L = nB
             const double logR = std::log(1e-6), Lb = (double)nB;
@@ -492,6 +496,7 @@ class Prop3D final : public Prop {
         a.nbytes = (uint32_t)(p[0].bytes());
         for (int q = 0; q < 8; ++q) { a.cz[q] = cz[q]; a.cx[q] = cx[q]; a.cy[q] = cy[q]; }
         a.w0 = w0;
+        a.w0lo = w0lo;
         for (int m = 0; m < 16; ++m) { a.wx[m] = wx[m]; a.wy[m] = wy[m]; }
         if (nB > 0) {
             a.zphi_in = zphi[phase].p; a.zphi_out = zphi[phase ^ 1].p; a.zpsi = zpsi.p; a.zbm1 = zbm1.p;
