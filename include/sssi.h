@@ -217,6 +217,48 @@ int sssi_rtm2d_get_shot(sssi_rtm *r, int shot, double *image, double *num, doubl
 int sssi_rtm2d_device_bytes(sssi_rtm *r, double *bytes);
 void sssi_rtm2d_destroy(sssi_rtm *r);
 
+/* ---------------------------------------------------------------------
+ * Serving: a request loop with continuous batching.  Jobs are submitted at
+ * any time (inputs are copied); a worker thread admits them when a slot is
+ * free and their device footprint fits the HBM budget, advances every
+ * resident job one time step per tick (the 2-D jobs of one nDiffOrder share
+ * one launch), keeps wavefields and checkpoints resident until the job ends
+ * and frees them when its result is copied out.  Results do not depend on
+ * co-resident jobs.  Job kinds: 2-D CPML forward (result: data (nx,nt)),
+ * 2-D CPML RTM shot on the given extended grid (source at depth 0, column
+ * src_ix; result: image (nz,nx) = M ./ (eps + s2)), 3-D CPML forward
+ * (result: data (nx,ny,nt)).  No counterpart in the reference (its shots run
+ * one after another in MATLAB loops, mainRtmTimeCpmlFor2dAw.m:147).
+ * ------------------------------------------------------------------- */
+typedef struct sssi_server sssi_server;
+
+int sssi_server_create(double hbm_budget_bytes, int max_active, sssi_server **out);
+int sssi_server_submit_fwd2d(sssi_server *s, const double *v, int64_t nz, int64_t nx, int64_t nt,
+                             int nDiffOrder, int nBoundary, double dz, double dx, double dt,
+                             int src_iz, int src_ix, const double *wavelet, int64_t *job_id);
+int sssi_server_submit_rtm2d(sssi_server *s, const double *V, const double *VS, int64_t nz,
+                             int64_t nx, int64_t nt, int nDiffOrder, int nBoundary, double dz,
+                             double dx, double dt, int src_ix, const double *wavelet,
+                             int img_stride, int ck_format, double eps, int64_t *job_id);
+int sssi_server_submit_fwd3d(sssi_server *s, const double *v, int64_t nz, int64_t nx, int64_t ny,
+                             int64_t nt, int nDiffOrder, int nBoundary, double dz, double dx,
+                             double dy, double dt, int src_iz, int src_ix, int src_iy,
+                             const double *wavelet, int64_t *job_id);
+/* Wait for a job (timeout_s < 0: no limit); *done = 1 when finished. */
+int sssi_server_wait(sssi_server *s, int64_t job_id, double timeout_s, int *done);
+/* Wait until every submitted job has finished. */
+int sssi_server_drain(sssi_server *s);
+/* Result of a finished job (n values, layout as stated above). */
+int sssi_server_result(sssi_server *s, int64_t job_id, double *out, int64_t n);
+/* Host timestamps (seconds, steady clock) of arrival, admission and
+ * completion, and the device bytes the job reserved. */
+int sssi_server_job_info(sssi_server *s, int64_t job_id, double *t_arrive, double *t_admit,
+                         double *t_done, double *bytes);
+/* Totals: cell updates, seconds the worker spent issuing work, launches, ticks. */
+int sssi_server_stats(sssi_server *s, double *cell_updates, double *busy_s, int64_t *launches,
+                      int64_t *ticks);
+void sssi_server_destroy(sssi_server *s);
+
 #ifdef __cplusplus
 }
 #endif

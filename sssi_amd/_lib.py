@@ -62,9 +62,22 @@ SIGNATURES = {
     "sssi_rtm2d_get_shot": [_vp, _int, _D, _D, _D, _D],
     "sssi_rtm2d_device_bytes": [_vp, _D],
     "sssi_rtm2d_destroy": [_vp],
+    "sssi_server_create": [_dbl, _int, ctypes.POINTER(_vp)],
+    "sssi_server_submit_fwd2d": [_vp, _D, _i64, _i64, _i64, _int, _int, _dbl, _dbl, _dbl, _int,
+                                 _int, _D, ctypes.POINTER(_i64)],
+    "sssi_server_submit_rtm2d": [_vp, _D, _D, _i64, _i64, _i64, _int, _int, _dbl, _dbl, _dbl,
+                                 _int, _D, _int, _int, _dbl, ctypes.POINTER(_i64)],
+    "sssi_server_submit_fwd3d": [_vp, _D, _i64, _i64, _i64, _i64, _int, _int, _dbl, _dbl, _dbl,
+                                 _dbl, _int, _int, _int, _D, ctypes.POINTER(_i64)],
+    "sssi_server_wait": [_vp, _i64, _dbl, _I],
+    "sssi_server_drain": [_vp],
+    "sssi_server_result": [_vp, _i64, _D, _i64],
+    "sssi_server_job_info": [_vp, _i64, _D, _D, _D, _D],
+    "sssi_server_stats": [_vp, _D, _D, ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    "sssi_server_destroy": [_vp],
 }
 _RESTYPE = {"sssi_last_error": ctypes.c_char_p, "sssi_prop_destroy": None,
-            "sssi_rtm2d_destroy": None}
+            "sssi_rtm2d_destroy": None, "sssi_server_destroy": None}
 
 _lib = None
 

@@ -39,6 +39,7 @@ class Prop {
     int N = 1, nB = 0;
     double dz = 0, dx = 0, dy = 0, dt = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false when the stream is lent by a server
     int device = 0;
 
     // point sources, sorted by tile; amplitudes [t][k] in sorted order
@@ -69,6 +70,27 @@ class Prop {
     virtual void alg_bytes(double *bytes, double *cells) = 0;
 
     void sync() { SSSI_HIP(hipStreamSynchronize(stream)); }
+
+    // Grow the receiver buffer to nt steps of `per_step` values, keeping what
+    // was recorded so far.
+    void ensure_rec(int64_t nt, int64_t per_step) {
+        const size_t need = (size_t)nt * per_step;
+        if (rec.n < need) {
+            DevBuf<float> nr(need);
+            SSSI_HIP(hipMemsetAsync(nr.p, 0, nr.bytes(), stream));
+            if (rec.p)
+                SSSI_HIP(hipMemcpyAsync(nr.p, rec.p, rec.bytes(), hipMemcpyDeviceToDevice, stream));
+            SSSI_HIP(hipStreamSynchronize(stream));
+            rec = std::move(nr);
+        }
+        if (rec_nt < nt) rec_nt = nt;
+    }
+    // Run on a stream owned by someone else (the serving loop).
+    void use_stream(hipStream_t s) {
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        stream = s;
+        own_stream = false;
+    }
 };
 
 Prop *make_prop2d(const double *v, int64_t nz, int64_t nx, int abc, int N, int nB, double dz,

@@ -121,7 +121,7 @@ class Prop2D final : public Prop {
     }
 
     ~Prop2D() override {
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 
     int64_t cells() const override { return nz * nx; }
@@ -174,17 +174,6 @@ class Prop2D final : public Prop {
         SSSI_HIP(hipMemcpy(surf.p, h.data(), surf.bytes(), hipMemcpyHostToDevice));
     }
 
-    void ensure_rec(int64_t nt) {
-        if (rec.n < (size_t)(nt * nx)) {
-            DevBuf<float> nr((size_t)nt * nx);
-            SSSI_HIP(hipMemsetAsync(nr.p, 0, nr.bytes(), stream));
-            if (rec.p)
-                SSSI_HIP(hipMemcpyAsync(nr.p, rec.p, rec.bytes(), hipMemcpyDeviceToDevice, stream));
-            SSSI_HIP(hipStreamSynchronize(stream));
-            rec = std::move(nr);
-        }
-        rec_nt = std::max(rec_nt, nt);
-    }
 
     // Descriptor for `count` steps from time index t_first (see the kernel's
     // t / parity mapping): the level current now sits at parity rel_0 & 1.
@@ -258,7 +247,7 @@ class Prop2D final : public Prop {
              LaunchStats *st) override {
         if (count < 0) fail(SSSI_EINVAL, "run: negative step count");
         if (count == 0) return;
-        if (record) ensure_rec(reverse ? t_first + 1 : t_first + count);
+        if (record) ensure_rec(reverse ? t_first + 1 : t_first + count, nx);
         if (g) {
             int step0;
             const Job2D J = make_job(t_first, reverse, record, sink, &step0);

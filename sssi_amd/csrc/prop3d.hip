@@ -433,7 +433,7 @@ class Prop3D final : public Prop {
     }
 
     ~Prop3D() override {
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 
     int64_t cells() const override { return nz * nx * ny; }
@@ -529,10 +529,7 @@ class Prop3D final : public Prop {
     void run(int64_t t_first, int64_t count, bool reverse, bool record, const SnapSink *sink,
              LaunchStats *st) override {
         if (reverse) fail(SSSI_ESTATE, "3-D handles run forward modelling only");
-        if (record && rec_nt < t_first + count) {
-            rec.alloc((size_t)(t_first + count) * nx * ny);
-            rec_nt = t_first + count;
-        }
+        if (record) ensure_rec(t_first + count, nx * ny);
         for (int64_t q = 0; q < count; ++q) {
             const int64_t t = t_first + q;
             float *snap = nullptr;
