@@ -175,6 +175,92 @@ def bench_rtm(args):
     }
 
 
+def serve_jobs(njobs, seed=7):
+    """Seeded job population: 70 % 2-D (forward or RTM shot), grids 200x200 ..
+    2000x4000, nDiffOrder 2..5, nt 500..3000; 30 % 3-D forward, 128^3 .. 256^3,
+    nt 500..1500; layered 1500-4500 m/s; Ricker 10-30 Hz."""
+    from sssi_amd import synth
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(njobs):
+        f = float(rng.uniform(10, 30))
+        if rng.random() < 0.7:
+            nz, nx = int(rng.integers(200, 2001)), int(rng.integers(200, 4001))
+            N = int(rng.integers(2, 6))
+            nt = int(rng.integers(500, 3001))
+            v = synth.layered_2d(nz, nx, 6, 1500.0, 4500.0, seed=seed * 1000 + k)
+            nB = 20
+            dt = 0.5 * 10.0 / (4500.0 * np.sqrt(2) * float(np.sum(np.abs(synth.dCoef(N)))))
+            w = synth.ricker(f, nt, dt)
+            if rng.random() < 0.5:
+                jobs.append(("fwd2d", dict(v=v, nt=nt, nDiffOrder=N, nBoundary=nB, dz=10.0,
+                                           dx=10.0, dt=dt, src_iz=0, src_ix=nx // 2, wavelet=w)))
+            else:
+                jobs.append(("rtm2d", dict(V=v, VS=synth.smooth_copy(v, 5), nt=nt, nDiffOrder=N,
+                                           nBoundary=nB, dz=10.0, dx=10.0, dt=dt, src_ix=nx // 2,
+                                           wavelet=w, img_stride=5, ck_format="bf16")))
+        else:
+            n = int(rng.integers(128, 257))
+            nt = int(rng.integers(500, 1501))
+            N = int(rng.integers(2, 6))
+            v = synth.cake_3d_dipping(n, n, n, 4, 1500.0, 4500.0, seed=seed * 1000 + k)
+            dt = 0.5 * 10.0 / (4500.0 * np.sqrt(3) * float(np.sum(np.abs(synth.dCoef(N)))))
+            jobs.append(("fwd3d", dict(v=v, nt=nt, nDiffOrder=N, nBoundary=16, dz=10.0, dx=10.0,
+                                       dy=10.0, dt=dt, src=(0, n // 2, n // 2),
+                                       wavelet=synth.ricker(f, nt, dt))))
+    return jobs
+
+
+def bench_serve(args):
+    """Poisson arrivals of the seeded job mix at a fixed rate; p50/p99 latency and
+    throughput with continuous batching vs one job at a time."""
+    import threading
+    import sssi_amd
+    jobs = serve_jobs(args.jobs)
+    rng = np.random.default_rng(11)
+    gaps = rng.exponential(1.0 / args.rate, size=len(jobs))
+    out = {}
+    for name, max_active in (("batched", 64), ("one_at_a_time", 1)):
+        with sssi_amd.Server(hbm_budget_bytes=args.budget_gb * 1e9, max_active=max_active) as s:
+            ids = []
+            t0 = time.perf_counter()
+
+            def feeder():
+                t = t0
+                for (kind, kw), g in zip(jobs, gaps):
+                    t += g
+                    d = t - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                    ids.append(getattr(s, "submit_" + kind)(**kw))
+
+            th = threading.Thread(target=feeder)
+            th.start()
+            th.join()
+            s.drain()
+            wall = time.perf_counter() - t0
+            lat = np.array([s.info(i)["latency"] for i in ids])
+            st = s.stats()
+        out[name] = {"p50_s": round(float(np.percentile(lat, 50)), 4),
+                     "p99_s": round(float(np.percentile(lat, 99)), 4),
+                     "jobs_per_s": round(len(jobs) / wall, 3),
+                     "gcell_updates_per_s": round(st["cell_updates"] / wall / 1e9, 2),
+                     "wall_s": round(wall, 3), "launches": st["launches"], "ticks": st["ticks"]}
+    b, p = out["batched"], out["one_at_a_time"]
+    return {
+        "metric": "serving p50/p99 latency and throughput (mixed 2D/3D forward + RTM jobs)",
+        "value": b["jobs_per_s"], "unit": "jobs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded layered models, Ricker sources, Poisson arrivals)",
+        "config": {"workload": f"{len(jobs)} jobs, Poisson rate {args.rate}/s, 70% 2D fwd/RTM "
+                               "200x200..2000x4000, 30% 3D fwd 128^3..256^3",
+                   "parallelism": "1 GPU"},
+        "p50_s": b["p50_s"], "p99_s": b["p99_s"],
+        "throughput_vs_one_at_a_time": round(b["jobs_per_s"] / p["jobs_per_s"], 3),
+        "modes": out,
+    }
+
+
 def bench_copy(args):
     """Device-to-device copy of a known size (hipMemcpy's blit kernel), the
     calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -205,11 +291,14 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "rtm", "copy"],
+    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "rtm", "serve", "copy"],
                     help="2d: the headline 2-D forward line (default); 3d / rtm: other workloads")
     ap.add_argument("--nt", type=int, default=None, help="3d: time steps (default 2000)")
     ap.add_argument("--shots", type=int, default=64, help="rtm: number of shots")
     ap.add_argument("--ck", default="bf16", choices=["fp16", "bf16"], help="rtm: checkpoint type")
+    ap.add_argument("--jobs", type=int, default=60, help="serve: number of jobs")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate (1/s)")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget (GB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are "
@@ -220,6 +309,9 @@ def main():
         return
     if args.workload == "rtm":
         print(json.dumps(bench_rtm(args)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
         return
     if args.workload == "copy":
         print(json.dumps(bench_copy(args)), flush=True)

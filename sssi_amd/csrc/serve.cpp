@@ -80,10 +80,14 @@ class Server {
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<ServeJob *> queue;
+    std::deque<ServeJob *> queue;       // submitted, waiting for a slot and memory
+    std::deque<ServeJob *> ready;       // device state built, waiting for the next tick
+    std::deque<ServeJob *> finished;    // results copied out, device state to release
     std::map<int64_t, std::unique_ptr<ServeJob>> jobs;
     std::vector<ServeJob *> active;
-    std::thread worker;
+    int reserved_slots = 0;             // jobs admitted (being built, ready or active)
+    std::condition_variable cv_helper;
+    std::thread worker, helper;
     std::atomic<bool> stop{false};
     int64_t next_id = 1;
     int64_t tick = 0;
@@ -107,6 +111,7 @@ class Server {
         SSSI_HIP(hipGetDevice(&device));
         SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         worker = std::thread([this] { loop(); });
+        helper = std::thread([this] { helper_loop(); });
     }
 
     ~Server() {
@@ -115,7 +120,9 @@ class Server {
             stop = true;
         }
         cv_work.notify_all();
+        cv_helper.notify_all();
         if (worker.joinable()) worker.join();
+        if (helper.joinable()) helper.join();
         for (auto &kv : jobs)
             if (kv.second->done_ev) (void)hipEventDestroy(kv.second->done_ev);
         active.clear();
@@ -138,7 +145,7 @@ class Server {
         const int64_t id = j->id;
         jobs[id] = std::move(j);
         queue.push_back(raw);
-        cv_work.notify_all();
+        cv_helper.notify_all();
         return id;
     }
 
@@ -167,17 +174,100 @@ class Server {
         return 3 * np * 4 + 3 * strips * 4 + (size_t)j.nt * j.nx * j.ny * 4;
     }
 
-    // ---- worker
+    // ---- helper thread: builds device state of admitted jobs (host-side
+    // preparation and uploads) and releases the state of finished ones, so the
+    // worker's launch stream never waits on them.
+    void helper_loop() {
+        try {
+            SSSI_HIP(hipSetDevice(device));
+            while (true) {
+                ServeJob *build = nullptr, *drop = nullptr;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv_helper.wait(lk, [&] {
+                        return stop || !finished.empty() ||
+                               (!queue.empty() && reserved_slots < max_active &&
+                                (double)(used + queue.front()->bytes) <= budget);
+                    });
+                    if (stop) break;
+                    if (!finished.empty()) {
+                        drop = finished.front();
+                        finished.pop_front();
+                    } else {
+                        build = queue.front();  // FIFO: later jobs wait for memory
+                        queue.pop_front();
+                        used += build->bytes;
+                        reserved_slots++;
+                    }
+                }
+                if (drop) {
+                    release(*drop);
+                    std::lock_guard<std::mutex> lk(mu);
+                    used -= drop->bytes;
+                    reserved_slots--;
+                    drop->state = kDone;
+                    cv_done.notify_all();
+                    cv_helper.notify_all();
+                    continue;
+                }
+                try {
+                    setup(*build);
+                    std::lock_guard<std::mutex> lk(mu);
+                    build->t_admit = now_s();
+                    ready.push_back(build);
+                    cv_work.notify_all();
+                } catch (const std::exception &e) {
+                    release(*build);
+                    std::lock_guard<std::mutex> lk(mu);
+                    build->state = kFailed;
+                    build->error = e.what();
+                    used -= build->bytes;
+                    reserved_slots--;
+                    cv_done.notify_all();
+                }
+            }
+        } catch (const std::exception &e) {
+            std::lock_guard<std::mutex> lk(mu);
+            fatal = e.what();
+            cv_done.notify_all();
+        }
+    }
+
+    void release(ServeJob &j) {
+        j.g.reset();
+        j.gs.reset();
+        j.p3.reset();
+        for (auto *b : {&j.rec, &j.rec2, &j.ck32, &j.stage, &j.num, &j.den, &j.img, &j.ckscale})
+            b->release();
+        j.ck16.release();
+        j.ckmax.release();
+        j.src = DevSources();
+        j.src_s = DevSources();
+        if (j.done_ev) {
+            (void)hipEventDestroy(j.done_ev);
+            j.done_ev = nullptr;
+        }
+    }
+
+    // ---- worker: one tick = one time step of every resident job
     void loop() {
         try {
             SSSI_HIP(hipSetDevice(device));
             while (true) {
                 {
                     std::unique_lock<std::mutex> lk(mu);
-                    cv_work.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+                    cv_work.wait(lk, [&] { return stop || !ready.empty() || !active.empty(); });
                     if (stop && active.empty()) break;
+                    while (!ready.empty()) {
+                        ServeJob *j = ready.front();
+                        ready.pop_front();
+                        j->state = kActive;
+                        active.push_back(j);
+                        groups_dirty = true;
+                    }
                 }
-                admit();
+                for (ServeJob *j : active)
+                    if (j->state == kActive && j->phase_start < 0) start_phase(*j);
                 if (active.empty()) continue;
                 const double t0 = now_s();
                 step_all();
@@ -196,48 +286,19 @@ class Server {
         }
     }
 
-    void admit() {
-        std::vector<ServeJob *> take;
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            while (!queue.empty() && (int)(active.size() + take.size()) < max_active) {
-                ServeJob *j = queue.front();
-                if ((double)(used + j->bytes) > budget) break;  // FIFO: wait for memory
-                queue.pop_front();
-                used += j->bytes;
-                take.push_back(j);
-            }
-        }
-        for (ServeJob *j : take) {
-            try {
-                setup(*j);
-                j->t_admit = now_s();
-                j->state = kActive;
-                active.push_back(j);
-                groups_dirty = true;
-            } catch (const std::exception &e) {
-                std::lock_guard<std::mutex> lk(mu);
-                j->state = kFailed;
-                j->error = e.what();
-                used -= j->bytes;
-                cv_done.notify_all();
-            }
-        }
-    }
-
     void setup(ServeJob &j) {
         if (j.kind == kFwd3d) {
             j.p3.reset(make_prop3d(j.v.data(), j.nz, j.nx, j.ny, j.N, j.nB, j.dz, j.dx, j.dy, j.dt));
             j.p3->use_stream(stream);
             j.p3->set_points(1, &j.src_iz, &j.src_ix, &j.src_iy, j.wavelet.data(), j.nt);
             j.p3->ensure_rec(j.nt, j.nx * j.ny);
-            j.phase_start = tick;
+            j.p3->sync();
+            j.phase_start = -1;
             return;
         }
         j.g.reset(new Cpml2dGrid(j.v.data(), j.nz, j.nx, j.N, j.nB, j.dz, j.dx, j.dt));
         j.src.set(*j.g, 1, &j.src_iz, &j.src_ix, j.wavelet.data(), j.nt);
         j.rec.alloc((size_t)j.nt * j.nx);
-        j.g->zero_state(stream);
         if (j.kind == kRtm2d) {
             j.gs.reset(new Cpml2dGrid(j.vs.data(), j.nz, j.nx, j.N, j.nB, j.dz, j.dx, j.dt));
             j.src_s.set(*j.gs, 1, &j.src_iz, &j.src_ix, j.wavelet.data(), j.nt);
@@ -257,11 +318,12 @@ class Server {
             j.img.alloc(cells);
         }
         j.phase = 0;
-        start_phase(j);
+        j.phase_start = -1;  // uploads above are synchronous; the worker zeroes state
     }
 
     void start_phase(ServeJob &j) {
         j.phase_start = tick;
+        if (j.kind == kFwd3d) return;
         Job2D J;
         if (j.kind == kFwd2d || (j.kind == kRtm2d && j.phase == 0)) {
             j.g->zero_state(stream);
@@ -393,25 +455,14 @@ class Server {
             changed = true;
         }
         if (changed) groups_dirty = true;
-        // completed jobs: release memory, record latency
+        // completed jobs (results on the host): the helper releases their memory
         std::vector<ServeJob *> keep;
         for (ServeJob *j : active) {
             if (j->state == kDraining && hipEventQuery(j->done_ev) == hipSuccess) {
-                j->g.reset();
-                j->gs.reset();
-                j->p3.reset();
-                for (auto *b : {&j->rec, &j->rec2, &j->ck32, &j->stage, &j->num, &j->den, &j->img,
-                                &j->ckscale})
-                    b->release();
-                j->ck16.release();
-                j->ckmax.release();
-                j->src = DevSources();
-                j->src_s = DevSources();
                 std::lock_guard<std::mutex> lk(mu);
-                j->t_done = now_s();
-                j->state = kDone;
-                used -= j->bytes;
-                cv_done.notify_all();
+                j->t_done = now_s();  // result is on the host
+                finished.push_back(j);
+                cv_helper.notify_all();
                 continue;
             }
             keep.push_back(j);
