@@ -1,0 +1,182 @@
+"""MEX gateways (sssi_amd/mex) compiled against a test double of the MEX API.
+
+CPU: every gateway compiles warning-free and rejects bad calls with a MATLAB
+error identifier before touching the device (argument count, class,
+dimensions, integer ranges, the reference's unchecked 2-D source, options).
+GPU: the gateway outputs equal the C-ABI drop-ins bit for bit, the struct
+(point) source equals the dense source, snapStride slices equal the full
+snapshot's, nargout = 1 skips the snapshot, and the gateways match the FP64
+oracle (rel-L2 <= 1e-5).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sssi_amd
+from sssi_amd import synth
+from tests import mexmock
+from tests.mexmock import MexError, call
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    mexmock.build()
+
+
+def small_2d(nt=120):
+    v = synth.layered_2d(40, 70, 3, 1500, 3000, seed=7)
+    w = synth.ricker(25, nt, 1e-3)
+    src = np.zeros((40, 70, nt), order="F")
+    src[0, 35, :] = w
+    return v, src, w
+
+
+@pytest.mark.parametrize("name,nargs", [("fwdTimeCpmlFor2dAw_mex", 7),
+                                        ("rvsTimeCpmlFor2dAw_mex", 7),
+                                        ("fwdTimeCpmlFor3dAw_mex", 8),
+                                        ("fwdTimeSpongeFor2dAw_mex", 6),
+                                        ("rvsTimeSpongeFor2dAw_mex", 6)])
+def test_argument_count(name, nargs):
+    with pytest.raises(MexError) as e:
+        call(name, *([1.0] * (nargs - 1)))
+    assert e.value.ident == "sssi:nargin"
+    with pytest.raises(MexError) as e:
+        call(name, *([1.0] * (nargs + 2)))
+    assert e.value.ident == "sssi:nargin"
+
+
+def test_fwd2d_validation():
+    v, src, _ = small_2d(10)
+    args = [v, src, 2, 10, 10.0, 10.0, 1e-3]
+    bad = list(args)
+    bad[0] = mexmock.Int32Scalar()
+    with pytest.raises(MexError) as e:
+        call("fwdTimeCpmlFor2dAw_mex", *bad)
+    assert e.value.ident == "sssi:class"
+    bad = list(args)
+    bad[1] = src[:, :-1, :]
+    with pytest.raises(MexError) as e:
+        call("fwdTimeCpmlFor2dAw_mex", *bad)
+    assert e.value.ident == "sssi:size"
+    for k, val in [(2, 9), (2, 0), (2, 2.5), (3, -1)]:
+        bad = list(args)
+        bad[k] = val
+        with pytest.raises(MexError) as e:
+            call("fwdTimeCpmlFor2dAw_mex", *bad)
+        assert e.value.ident == "sssi:range"
+    with pytest.raises(MexError) as e:
+        call("fwdTimeCpmlFor2dAw_mex", *args, {"snapStride": 2.0})
+    assert e.value.ident == "sssi:opts"
+    with pytest.raises(MexError) as e:
+        call("fwdTimeCpmlFor2dAw_mex", *args, {"snapStride": 0.0})
+    assert e.value.ident == "sssi:range"
+    # library-side validation (boundary thicker than the model) surfaces as sssi:run
+    bad = list(args)
+    bad[3] = 40
+    with pytest.raises(MexError) as e:
+        call("fwdTimeCpmlFor2dAw_mex", *bad)
+    assert e.value.ident == "sssi:run" and "nBoundary" in e.value.msg
+
+
+def test_struct_source_validation():
+    v, _, w = small_2d(10)
+    with pytest.raises(MexError) as e:
+        call("fwdTimeCpmlFor2dAw_mex", v, {"iz": [0.0], "wavelet": w}, 2, 10, 10.0, 10.0, 1e-3)
+    assert e.value.ident == "sssi:source" and "ix" in e.value.msg
+    with pytest.raises(MexError) as e:
+        call("fwdTimeCpmlFor2dAw_mex", v, {"iz": [0.0, 1.0], "ix": [3.0, 4.0], "wavelet": w},
+             2, 10, 10.0, 10.0, 1e-3)
+    assert e.value.ident == "sssi:source"
+    v3 = np.full((8, 9, 10), 2000.0)
+    with pytest.raises(MexError) as e:
+        call("fwdTimeCpmlFor3dAw_mex", v3, {"iz": [0.0], "ix": [3.0], "wavelet": w},
+             2, 2, 10.0, 10.0, 10.0, 1e-3)
+    assert e.value.ident == "sssi:source" and "iy" in e.value.msg
+
+
+def test_rvs_validation():
+    v, _, _ = small_2d(10)
+    with pytest.raises(MexError) as e:
+        call("rvsTimeCpmlFor2dAw_mex", v, np.zeros((69, 10)), 2, 10, 10.0, 10.0, 1e-3)
+    assert e.value.ident == "sssi:size"
+    with pytest.raises(MexError) as e:
+        call("rvsTimeSpongeFor2dAw_mex", v, np.zeros((71, 10)), 10, 10.0, 10.0, 1e-3)
+    assert e.value.ident == "sssi:size"
+
+
+@pytest.mark.gpu
+def test_fwd2d_gateway_equals_abi_and_oracle():
+    v, src, w = small_2d()
+    d_mex, s_mex = call("fwdTimeCpmlFor2dAw_mex", v, src, 3, 10, 10.0, 10.0, 1e-3)
+    d_api, s_api = sssi_amd.fwdTimeCpmlFor2dAw(v, src, 3, 10, 10.0, 10.0, 1e-3)
+    assert d_mex.shape == (70, 120) and s_mex.shape == (40, 70, 120)
+    assert np.array_equal(d_mex, d_api) and np.array_equal(s_mex, s_api)
+    d_o, s_o = oracle.fwd_cpml_2d(v, 120, 3, 10, 10.0, 10.0, 1e-3, source=src,
+                                  want_snapshot=True)
+    assert rel_l2(d_mex, d_o) < 1e-5 and rel_l2(s_mex, s_o) < 1e-5
+    # point source, snapshot stride, nargout = 1
+    pts = {"iz": [0.0], "ix": [35.0], "wavelet": w.reshape(-1, 1)}
+    d_p, s_p = call("fwdTimeCpmlFor2dAw_mex", v, pts, 3, 10, 10.0, 10.0, 1e-3,
+                    {"snapStride": 7.0})
+    assert np.array_equal(d_p, d_mex)
+    assert s_p.shape == (40, 70, 18) and np.array_equal(s_p, s_mex[:, :, ::7])
+    d1, s1 = call("fwdTimeCpmlFor2dAw_mex", v, src, 3, 10, 10.0, 10.0, 1e-3, nargout=1)
+    assert s1 is None and np.array_equal(d1, d_mex)
+
+
+@pytest.mark.gpu
+def test_fwd2d_gateway_two_dimensional_source_is_one_step():
+    v, src, _ = small_2d(1)
+    d2, s2 = call("fwdTimeCpmlFor2dAw_mex", v, src[:, :, 0], 2, 10, 10.0, 10.0, 1e-3)
+    d3, s3 = sssi_amd.fwdTimeCpmlFor2dAw(v, src, 2, 10, 10.0, 10.0, 1e-3)
+    assert d2.shape == (70, 1) and s2.shape == (40, 70)
+    assert np.array_equal(d2, d3) and np.array_equal(s2, s3[:, :, 0])
+
+
+@pytest.mark.gpu
+def test_rvs2d_gateway_equals_abi():
+    v, src, _ = small_2d()
+    data, _ = sssi_amd.fwdTimeCpmlFor2dAw(v, src, 3, 10, 10.0, 10.0, 1e-3, nargout=1)
+    m_mex, s_mex = call("rvsTimeCpmlFor2dAw_mex", v, data, 3, 10, 10.0, 10.0, 1e-3)
+    m_api, s_api = sssi_amd.rvsTimeCpmlFor2dAw(v, data, 3, 10, 10.0, 10.0, 1e-3)
+    assert m_mex.shape == (40 + 10, 70 + 10)
+    assert np.array_equal(m_mex, m_api) and np.array_equal(s_mex, s_api)
+    m_s, s_s = call("rvsTimeCpmlFor2dAw_mex", v, data, 3, 10, 10.0, 10.0, 1e-3,
+                    {"snapStride": 5.0})
+    assert np.array_equal(m_s, m_mex) and np.array_equal(s_s, s_mex[:, :, ::5])
+
+
+@pytest.mark.gpu
+def test_sponge_gateways_equal_abi():
+    v, src, _ = small_2d()
+    d_mex, s_mex = call("fwdTimeSpongeFor2dAw_mex", v, src, 10, 10.0, 10.0, 1e-3)
+    d_api, s_api = sssi_amd.fwdTimeSpongeFor2dAw(v, src, 10, 10.0, 10.0, 1e-3)
+    assert np.array_equal(d_mex, d_api) and np.array_equal(s_mex, s_api)
+    m_mex, r_mex = call("rvsTimeSpongeFor2dAw_mex", v, d_mex, 10, 10.0, 10.0, 1e-3)
+    m_api, r_api = sssi_amd.rvsTimeSpongeFor2dAw(v, d_mex, 10, 10.0, 10.0, 1e-3)
+    assert m_mex.shape == (42, 72)
+    assert np.array_equal(m_mex, m_api) and np.array_equal(r_mex, r_api)
+
+
+@pytest.mark.gpu
+def test_fwd3d_gateway_equals_abi_and_oracle():
+    v = synth.cake_3d_dipping(20, 24, 22, 3, 1500, 2500, seed=3)
+    nt = 60
+    w = synth.ricker(30, nt, 1e-3)
+    src = np.zeros((20, 24, 22, nt), order="F")
+    src[0, 12, 11, :] = w
+    d_mex, s_mex = call("fwdTimeCpmlFor3dAw_mex", v, src, 2, 5, 10.0, 10.0, 10.0, 1e-3)
+    d_api, s_api = sssi_amd.fwdTimeCpmlFor3dAw(v, src, 2, 5, 10.0, 10.0, 10.0, 1e-3)
+    assert d_mex.shape == (24, 22, nt) and s_mex.shape == (20, 24, 22, nt)
+    assert np.array_equal(d_mex, d_api) and np.array_equal(s_mex, s_api)
+    d_o, _ = oracle.fwd_cpml_3d(v, nt, 2, 5, 10.0, 10.0, 10.0, 1e-3, src_iz=[0], src_ix=[12],
+                                src_iy=[11], wavelet=w)
+    assert rel_l2(d_mex, d_o) < 1e-5
+    pts = {"iz": [0.0], "ix": [12.0], "iy": [11.0], "wavelet": w.reshape(-1, 1)}
+    d_p, s_p = call("fwdTimeCpmlFor3dAw_mex", v, pts, 2, 5, 10.0, 10.0, 10.0, 1e-3,
+                    {"snapStride": 4.0})
+    assert np.array_equal(d_p, d_mex) and np.array_equal(s_p, s_mex[..., ::4])
