@@ -21,6 +21,9 @@ constexpr int kCkSlots = 64;  // per-checkpoint |max| slots (spread the atomics)
 struct Job2D {
     int nz, nx, pitch, rows, ntx, nwz, nB;
     int start_step, nt, reverse;  // t = start..: forward t = rel, reverse t = nt-1-rel
+    // interior march regions: rows [0, mz_rows) x column tiles [mta, mta+mntx),
+    // blocks of 4 column tiles (mnxb per row band) x (march groups * rz) rows
+    int mz_rows, mta, mntx, mnxb;
     float cz[8], cx[8];           // c_k/dz, c_k/dx (two-stage form)
     float w0, w0lo, wz[16], wx[16];  // composite D-D+ weights / d^2 (w0 + w0lo: centre)
     float *p[2];                  // wavefield levels (padded); parity picks current
@@ -60,19 +63,27 @@ class Cpml2dGrid {
    public:
     int64_t nz = 0, nx = 0, pitch = 0, rows = 0;
     int N = 1, nB = 0, rz = 4;
+    // interior march: ng groups of rz rows per wave (0: tile path only), over
+    // rows [0, mz_rows) and column tiles [mta, mta + mntx)
+    int ng = 0, mz_rows = 0, mta = 0, mntx = 0;
     double dz = 0, dx = 0, dt = 0;
     DevBuf<float> p[2], vdt2;
     DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1;
     float cz[8] = {0}, cx[8] = {0}, w0 = 0.f, w0lo = 0.f, wz[16] = {0}, wx[16] = {0};
 
+    // rz 0 / ng -1: the SSSI_RZ / SSSI_NG environment variables or the defaults
     Cpml2dGrid(const double *v, int64_t nz, int64_t nx, int N, int nB, double dz, double dx,
-               double dt, int rz = 0);
+               double dt, int rz = 0, int ng = -1);
     int ntx() const { return (int)((nx + kTX - 1) / kTX); }
     int nwz() const { return (int)((nz + rz - 1) / rz); }
     int tz() const { return 4 * rz; }
     int ntz() const { return (int)((nz + tz() - 1) / tz()); }
     int nblocks() const { return ntx() * ntz(); }
     bool block_is_strip(int tile) const;
+    // tile block covered by the interior march (not launched as a tile block)
+    bool block_in_march(int tile) const;
+    int march_nxb() const { return (mntx + 3) / 4; }
+    int n_march_blocks() const { return mntx > 0 ? (mz_rows / (ng * rz)) * march_nxb() : 0; }
     void zero_state(hipStream_t s);
     // descriptor with geometry, coefficients and state pointers filled in
     Job2D job() const;
@@ -90,17 +101,24 @@ struct DevSources {
              int64_t nt);
 };
 
-// Block launch order for a set of jobs: (job, block tile) pairs with the
-// strip tiles of every job first, interior tiles split into 8 contiguous bands
-// (block b is dispatched to XCD b % 8).
+// Interior march groups (rz rows each) per wave for a tile height rz, from
+// the environment variable SSSI_NG (default 0: tile path only).
+int march_groups(int rz);
+
+// Block launch order for a set of jobs: (job, block) pairs; block >= 0 is a
+// 4-tile block of the tile path, block < 0 the interior march block -1-block.
+// Interior march blocks of every job come first, then the strip tiles, then
+// the remaining interior tiles; the march blocks and interior tiles are each
+// split into 8 contiguous bands (block b is dispatched to XCD b % 8).
 void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector<int2> &order);
 
 // Launch one step of a single job (descriptor by value).
 void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
                           int step, hipStream_t s);
-// Launch one step of many jobs sharing nDiffOrder and rz (descriptors in device memory).
-void launch_cpml2d_batch(int N, int rz, const Job2D *jobs, const int2 *order, int nblk, int step,
-                         hipStream_t s);
+// Launch one step of many jobs sharing nDiffOrder, rz and march layout
+// (descriptors in device memory).
+void launch_cpml2d_batch(int N, int rz, int ng, const Job2D *jobs, const int2 *order, int nblk,
+                         int step, hipStream_t s);
 // Pack the staged FP32 checkpoint `c` of a job into 16-bit (after its last step).
 void launch_ck_pack(const Job2D &job, int c, hipStream_t s);
 

@@ -44,6 +44,22 @@ __device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
 }
 
+// Load of wave-uniform data that no launch of this kernel writes (source
+// tables, checkpoint scales of earlier steps): through the constant address
+// space the compiler emits a scalar load, which is waited for with lgkmcnt and
+// leaves the in-order vmcnt of the prefetched field loads untouched.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
+// Index of the wavefront in its block, as a wave-uniform (scalar) value: the
+// compiler cannot prove threadIdx.x >> 6 uniform, and a row offset derived
+// from it would turn every buffer load into a waterfall loop.
+__device__ __forceinline__ int wave_id() {
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // LDS written by some lanes of a wavefront and read by others: wavefront-scope
 // release/acquire (no workgroup barrier; waves of a block are independent).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -63,31 +79,205 @@ __device__ __forceinline__ float f16_to_32(uint16_t h, int fmt) {
     return __uint_as_float((uint32_t)h << 16);
 }
 
+// Wave-uniform quantities of one time step of one job.
+struct StepCtx {
+    int t, par;
+    uint32_t rowb, nbytes;
+    bool snap_now, ck_now, stage_now, pack_now, im_now;
+    int c, ci_im;
+    float scale, inv_scale, im_scale;
+};
+
+__device__ __forceinline__ StepCtx step_ctx(const Job2D &J, int step) {
+    StepCtx S;
+    const int rel = step - J.start_step;
+    S.t = J.reverse ? J.nt - 1 - rel : rel;
+    S.par = rel & 1;
+    S.rowb = (uint32_t)J.pitch * 4u;
+    S.nbytes = S.rowb * (uint32_t)J.rows;
+    S.snap_now = J.snap && (S.t % J.snap_stride == 0);
+    S.ck_now = J.ck_fmt != kCkNone && (S.t % J.ck_stride == 0);
+    S.c = S.ck_now ? S.t / J.ck_stride : 0;
+    S.stage_now = S.ck_now && J.ck_fmt != kCkF32;
+    S.pack_now = S.stage_now && S.c > 0;
+    S.im_now = J.im_num && (S.t % J.im_stride == 0);
+    S.ci_im = S.im_now ? S.t / J.im_stride : 0;
+    S.scale = S.inv_scale = S.im_scale = 1.f;
+    if (S.pack_now) {
+        // scale of checkpoint c-1 from its per-wave maxima (complete: written
+        // by the previous launch)
+        const unsigned *mx = J.ck_max + (size_t)(S.c - 1) * kCkSlots;
+        unsigned m = 0;
+        for (int q = 0; q < kCkSlots; ++q) m = max(m, ld_uniform(mx + q));
+        const float amax = __uint_as_float(m);
+        S.scale = amax > 0.f ? amax * (1.0f / 32768.0f) : 1.0f;
+        S.inv_scale = 1.0f / S.scale;
+    }
+    if (S.im_now && J.im_fmt != kCkF32) S.im_scale = ld_uniform(J.im_scale + S.ci_im);
+    return S;
+}
+
+// Source terms of the RZ rows zs.. of column j: forward point sources from the
+// per-wave-tile CSR, reverse surface injection at row 0.
+template <int RZ>
+__device__ __forceinline__ void tile_sources(const Job2D &J, int t, int zs, int txi, int j,
+                                             bool jin, float (&s)[RZ]) {
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) s[r] = 0.f;
+    if (!J.reverse && J.src_ptr && t < J.src_nt) {
+        const int wt = (zs / RZ) * J.ntx + txi;
+        const int k0 = ld_uniform(J.src_ptr + wt), k1 = ld_uniform(J.src_ptr + wt + 1);
+        const float *amp = J.src_amp + (size_t)t * J.nsrc;
+        for (int k = k0; k < k1; ++k) {
+            const int cell = ld_uniform(J.src_cell + k);
+            const int ci = cell / J.nx, cj = cell - ci * J.nx;
+            const float a = ld_uniform(amp + k);
+            if (cj == j) {
+#pragma unroll
+                for (int r = 0; r < RZ; ++r)
+                    if (ci == zs + r) s[r] += a;
+            }
+        }
+    }
+    if (J.reverse && J.surf && zs == 0 && jin && t < J.surf_nt) {
+        const size_t o = (size_t)t * J.nx + j;
+        s[0] += J.surf2 ? J.surf[o] - J.surf2[o] : J.surf[o];
+    }
+}
+
+// p_new = v^2dt^2 (P_z + P_x + s) + 2p - p_old for the RZ rows zs.. of column
+// j, stored over p_old, then receivers, snapshot, checkpoint and imaging.
+template <int RZ>
+__device__ __forceinline__ void tile_update(const Job2D &J, const StepCtx &S,
+                                            __amdgpu_buffer_rsrc_t rO, uint32_t vcol,
+                                            uint32_t srow, int zs, int j, bool jin,
+                                            const float (&pc)[RZ], const float (&po)[RZ],
+                                            const float (&v2)[RZ], const float (&lap)[RZ],
+                                            const float (&s)[RZ], float &wmax) {
+    const size_t ncell = (size_t)J.nz * J.nx;
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+        const int i = zs + r;
+        if (jin && i < J.nz) {
+            // pc + (v2*(...) + (pc - po)): pc - po is exact when the two levels are
+            // within a factor 2, which keeps the rounding of the update at the
+            // scale of the increment rather than of the field.
+            const float pn = pc[r] + fmaf(v2[r], lap[r] + s[r], pc[r] - po[r]);
+            buf_st(pn, rO, vcol, srow + r * S.rowb);
+            const size_t cell = (size_t)i * J.nx + j;
+            if (J.rec && i == 0) J.rec[(size_t)S.t * J.nx + j] = pn;
+            if (S.snap_now)
+                J.snap[(size_t)(S.t / J.snap_stride - J.snap_slot0) * ncell + cell] = pn;
+            if (S.ck_now) {
+                if (J.ck_fmt == kCkF32) {
+                    J.ck32[(size_t)S.c * ncell + cell] = pn;
+                } else {
+                    J.ck_stage[(size_t)(S.c & 1) * ncell + cell] = pn;
+                    wmax = fmaxf(wmax, fabsf(pn));
+                }
+            }
+            if (S.pack_now) {
+                const float prev = J.ck_stage[(size_t)((S.c - 1) & 1) * ncell + cell];
+                J.ck16[(size_t)(S.c - 1) * ncell + cell] = f32_to_16(prev * S.inv_scale, J.ck_fmt);
+            }
+            if (S.im_now) {
+                const size_t o = (size_t)S.ci_im * ncell + cell;
+                const float f = J.im_fmt == kCkF32
+                                    ? J.im_ck32[o]
+                                    : f16_to_32(J.im_ck16[o], J.im_fmt) * S.im_scale;
+                J.im_num[cell] = fmaf(f, pn, J.im_num[cell]);
+                J.im_den[cell] = fmaf(f, f, J.im_den[cell]);
+            }
+        }
+    }
+}
+
+// Materialise values here: the compiler would otherwise sink their arithmetic
+// past the branches of the source and epilogue code and keep every operand
+// (the LDS neighbourhood of all rows) live across them.
+template <int R>
+__device__ __forceinline__ void pin(float (&v)[R]) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) asm volatile("" : "+v"(v[r]));
+}
+
+// Per-wave |p| maximum of a staged checkpoint into one of the kCkSlots slots.
+__device__ __forceinline__ void ck_max_commit(const Job2D &J, const StepCtx &S, float wmax,
+                                              int slot, int lane) {
+    if (!S.stage_now) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, off, 64));
+    if (lane == 0)
+        atomicMax(J.ck_max + (size_t)S.c * kCkSlots + (slot & (kCkSlots - 1)),
+                  __float_as_uint(wmax));
+}
+
+// Composite D-D+ Laplacian of RZ rows: the z window win[OFF + r] holds row
+// zs - L + r (r < RZ + 2L), the wave's LDS rows sp[r * PW + L + lane] the x
+// neighbourhood of row zs + r.  OFF is a constant after unrolling.
+template <int N, int RZ, int W>
+__device__ __forceinline__ void interior_lap(const Job2D &J, const float (&win)[W], int OFF,
+                                             const float *sp, int lane, float (&lap)[RZ]) {
+    constexpr int L = 2 * N - 1;
+    constexpr int PW = kTX + 2 * L;
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+        const float *row = sp + r * PW + L + lane;
+        float acc = J.w0 * win[OFF + L + r];
+#pragma unroll
+        for (int m = 1; m <= L; ++m)
+            acc = fmaf(J.wz[m], win[OFF + L + r + m] + win[OFF + L + r - m], acc);
+#pragma unroll
+        for (int m = 1; m <= L; ++m) acc = fmaf(J.wx[m], row[m] + row[-m], acc);
+        // the low part of the centre weight, added once the large terms have
+        // cancelled, makes the weights sum to zero to FP32 resolution of lap
+        lap[r] = fmaf(J.w0lo, win[OFF + L + r], acc);
+    }
+    pin(lap);
+}
+
+// x halo of RZ rows (L columns each side of the 64-wide tile): lane e of
+// load u fetches value e + 64u; h_lds is its LDS slot (or -1).
 template <int N, int RZ>
-__device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step) {
+__device__ __forceinline__ void halo_slots(int lane, int (&h_lds)[(2 * (2 * N - 1) * RZ + 63) / 64],
+                                           uint32_t (&h_off)[(2 * (2 * N - 1) * RZ + 63) / 64],
+                                           int x0, int pitch) {
+    constexpr int L = 2 * N - 1;
+    constexpr int PW = kTX + 2 * L;
+    constexpr int HN = 2 * L * RZ;
+    constexpr int HPL = (HN + 63) / 64;
+#pragma unroll
+    for (int u = 0; u < HPL; ++u) {
+        const int e = lane + 64 * u;
+        const int r = e / (2 * L), c = e - r * (2 * L);
+        const int xc = c < L ? x0 - L + c : x0 + kTX - L + c;
+        h_lds[u] = e < HN ? r * PW + (c < L ? c : kTX + c) : -1;
+        h_off[u] = (uint32_t)(r * pitch + xc + kHX) * 4u;
+    }
+}
+
+template <int N, int RZ>
+__device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, float *lds_p,
+                                            float *lds_a) {
     constexpr int L = 2 * N - 1;
     constexpr int PW = kTX + 2 * L;  // p rows: columns x0-L .. x0+63+L
     constexpr int AW = kTX + L;      // A_x rows: half points x0-N .. x0+63+N-1
     constexpr int HN = 2 * L * RZ;   // x-halo values per tile
     constexpr int HPL = (HN + 63) / 64;
-    __shared__ float s_p[4][RZ * PW];
-    __shared__ float s_a[4][RZ * AW];
 
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = wave_id();
     const int ntx = J.ntx;
     const int tzi = tile / ntx, txi = tile - tzi * ntx;
     const int x0 = txi * kTX;
     const int nz = J.nz, nx = J.nx, nB = J.nB;
     const int zs = (tzi * 4 + w) * RZ;
     if (zs >= nz) return;
-    const int rel = step - J.start_step;
-    const int t = J.reverse ? J.nt - 1 - rel : rel;
-    const int par = rel & 1;
-    const uint32_t rowb = (uint32_t)J.pitch * 4u;
-    const uint32_t nbytes = rowb * (uint32_t)J.rows;
-    const __amdgpu_buffer_rsrc_t rP = buf_rsrc(J.p[par], nbytes);
-    const __amdgpu_buffer_rsrc_t rO = buf_rsrc(J.p[par ^ 1], nbytes);
-    const __amdgpu_buffer_rsrc_t rV = buf_rsrc(J.vdt2, nbytes);
+    const StepCtx S = step_ctx(J, step);
+    const int t = S.t, par = S.par;
+    const uint32_t rowb = S.rowb;
+    const __amdgpu_buffer_rsrc_t rP = buf_rsrc(J.p[par], S.nbytes);
+    const __amdgpu_buffer_rsrc_t rO = buf_rsrc(J.p[par ^ 1], S.nbytes);
+    const __amdgpu_buffer_rsrc_t rV = buf_rsrc(J.vdt2, S.nbytes);
     const int j = x0 + lane;
     const bool jin = j < nx;
     const uint32_t vcol = (uint32_t)(j + kHX) * 4u;
@@ -103,17 +293,13 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step) 
         v2[r] = buf_ld(rV, vcol, srow + r * rowb);
     }
     int h_lds[HPL];
+    uint32_t h_off[HPL];
+    halo_slots<N, RZ>(lane, h_lds, h_off, x0, J.pitch);
 #pragma unroll
-    for (int u = 0; u < HPL; ++u) {
-        const int e = lane + 64 * u;
-        const int r = e / (2 * L), c = e - r * (2 * L);
-        const int xc = c < L ? x0 - L + c : x0 + kTX - L + c;
-        h_lds[u] = e < HN ? r * PW + (c < L ? c : kTX + c) : -1;
-        hal[u] = buf_ld(rP, (uint32_t)(r * J.pitch + xc + kHX) * 4u, srow);
-    }
+    for (int u = 0; u < HPL; ++u) hal[u] = buf_ld(rP, h_off[u], srow);
 
     // ---- the tile's rows with an L-wide x halo into the wave's LDS rows
-    float *sp = s_p[w];
+    float *sp = lds_p + w * (RZ * PW);
 #pragma unroll
     for (int r = 0; r < RZ; ++r) sp[r * PW + L + lane] = col[L + r];
 #pragma unroll
@@ -126,19 +312,7 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step) 
     const bool xstrip = nB > 0 && (x0 - N < nB || x0 + kTX + N - 1 > nx - nB);
     float lap[RZ];
     if (!zstrip && !xstrip) {
-        // ---- interior: composite stencil
-#pragma unroll
-        for (int r = 0; r < RZ; ++r) {
-            const float *row = sp + r * PW + L + lane;
-            float acc = J.w0 * col[L + r];
-#pragma unroll
-            for (int m = 1; m <= L; ++m) acc = fmaf(J.wz[m], col[L + r + m] + col[L + r - m], acc);
-#pragma unroll
-            for (int m = 1; m <= L; ++m) acc = fmaf(J.wx[m], row[m] + row[-m], acc);
-            // the low part of the centre weight, added once the large terms have
-            // cancelled, makes the weights sum to zero to FP32 resolution of lap
-            lap[r] = fmaf(J.w0lo, col[L + r], acc);
-        }
+        interior_lap<N, RZ>(J, col, 0, sp, lane, lap);
     } else {
         // ---- damping strips: two-stage form with Phi/Psi
         const float *zphi_in = J.zphi[par];
@@ -204,7 +378,7 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step) 
         // A_x at half points h = x0-N+q, q < 64+L: lane q, lanes < L also q = 64+lane
         const float *xphi_in = J.xphi[par];
         float *xphi_out = J.xphi[par ^ 1];
-        float *sa = s_a[w];
+        float *sa = lds_a + w * (RZ * AW);
         const int nb2 = 2 * nB;
         const bool xown = xstrip && jin && (j < nB || j >= nx - nB);
         const int xs = j < nB ? j : j - (nx - nb2);
@@ -267,105 +441,125 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step) 
         }
     }
 
-    // ---- sources
-    float s[RZ];
+    pin(lap);
+    float s[RZ], pc[RZ];
+    tile_sources<RZ>(J, t, zs, txi, j, jin, s);
 #pragma unroll
-    for (int r = 0; r < RZ; ++r) s[r] = 0.f;
-    if (!J.reverse && J.src_ptr && t < J.src_nt) {
-        const int wt = (zs / RZ) * ntx + txi;
-        const int k0 = J.src_ptr[wt], k1 = J.src_ptr[wt + 1];
-        const float *amp = J.src_amp + (size_t)t * J.nsrc;
-        for (int k = k0; k < k1; ++k) {
-            const int cell = J.src_cell[k];
-            const int ci = cell / nx, cj = cell - ci * nx;
-            if (cj == j) {
-#pragma unroll
-                for (int r = 0; r < RZ; ++r)
-                    if (ci == zs + r) s[r] += amp[k];
-            }
-        }
-    }
-    if (J.reverse && J.surf && zs == 0 && jin && t < J.surf_nt) {
-        const size_t o = (size_t)t * nx + j;
-        s[0] += J.surf2 ? J.surf[o] - J.surf2[o] : J.surf[o];
-    }
-
-    // ---- update p_new = v^2dt^2 (P_z + P_x + s) + 2p - p_old, and epilogues
-    const size_t ncell = (size_t)nz * nx;
-    const bool snap_now = J.snap && (t % J.snap_stride == 0);
-    const bool ck_now = J.ck_fmt != kCkNone && (t % J.ck_stride == 0);
-    const int c = ck_now ? t / J.ck_stride : 0;
-    const bool stage_now = ck_now && J.ck_fmt != kCkF32;
-    const bool pack_now = stage_now && c > 0;
-    const bool im_now = J.im_num && (t % J.im_stride == 0);
-    const int ci_im = im_now ? t / J.im_stride : 0;
-    float inv_scale = 1.f, im_scale = 1.f;
-    if (pack_now) {
-        // scale of checkpoint c-1 from its per-wave maxima
-        const unsigned *mx = J.ck_max + (size_t)(c - 1) * kCkSlots;
-        unsigned m = 0;
-        for (int q = 0; q < kCkSlots; ++q) m = max(m, mx[q]);
-        const float amax = __uint_as_float(m);
-        const float scale = amax > 0.f ? amax * (1.0f / 32768.0f) : 1.0f;
-        inv_scale = 1.0f / scale;
-        if (tile == 0 && w == 0 && lane == 0) J.ck_scale[c - 1] = scale;
-    }
-    if (im_now && J.im_fmt != kCkF32) im_scale = J.im_scale[ci_im];
+    for (int r = 0; r < RZ; ++r) pc[r] = col[L + r];
+    if (S.pack_now && tile == 0 && w == 0 && lane == 0) J.ck_scale[S.c - 1] = S.scale;
     float wmax = 0.f;
-#pragma unroll
-    for (int r = 0; r < RZ; ++r) {
-        const int i = zs + r;
-        if (jin && i < nz) {
-            const float pc = col[r + L];
-            // pc + (v2*(...) + (pc - po)): pc - po is exact when the two levels are
-            // within a factor 2, which keeps the rounding of the update at the
-            // scale of the increment rather than of the field.
-            const float pn = pc + fmaf(v2[r], lap[r] + s[r], pc - po[r]);
-            buf_st(pn, rO, vcol, srow + r * rowb);
-            const size_t cell = (size_t)i * nx + j;
-            if (J.rec && i == 0) J.rec[(size_t)t * nx + j] = pn;
-            if (snap_now) J.snap[(size_t)(t / J.snap_stride - J.snap_slot0) * ncell + cell] = pn;
-            if (ck_now) {
-                if (J.ck_fmt == kCkF32) {
-                    J.ck32[(size_t)c * ncell + cell] = pn;
-                } else {
-                    J.ck_stage[(size_t)(c & 1) * ncell + cell] = pn;
-                    wmax = fmaxf(wmax, fabsf(pn));
-                }
-            }
-            if (pack_now) {
-                const float prev = J.ck_stage[(size_t)((c - 1) & 1) * ncell + cell];
-                J.ck16[(size_t)(c - 1) * ncell + cell] = f32_to_16(prev * inv_scale, J.ck_fmt);
-            }
-            if (im_now) {
-                const size_t o = (size_t)ci_im * ncell + cell;
-                const float f = J.im_fmt == kCkF32 ? J.im_ck32[o]
-                                                   : f16_to_32(J.im_ck16[o], J.im_fmt) * im_scale;
-                J.im_num[cell] = fmaf(f, pn, J.im_num[cell]);
-                J.im_den[cell] = fmaf(f, f, J.im_den[cell]);
-            }
-        }
-    }
-    if (stage_now) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, off, 64));
-        if (lane == 0)
-            atomicMax(J.ck_max + (size_t)c * kCkSlots + ((tile * 4 + w) & (kCkSlots - 1)),
-                      __float_as_uint(wmax));
-    }
+    tile_update<RZ>(J, S, rO, vcol, srow, zs, j, jin, pc, po, v2, lap, s, wmax);
+    ck_max_commit(J, S, wmax, tile * 4 + w, lane);
 }
 
-template <int N, int RZ>
+// Interior march: a wavefront owns 64 columns x NG*RZ rows of the interior
+// (no damping strip within stencil reach, every cell inside the model) and
+// walks down them RZ rows at a time.  The z window stays in registers from
+// one group to the next (each p row is loaded once per wave instead of
+// (RZ + 2L) / RZ times), and the loads of group g+1 are issued before group g
+// is computed, so every wave keeps memory requests in flight while it
+// computes.  Arithmetic is that of the tile path's interior branch.
+template <int N, int RZ, int NG>
+__device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, float *lds_p) {
+    constexpr int L = 2 * N - 1;
+    constexpr int PW = kTX + 2 * L;
+    constexpr int HPL = (2 * L * RZ + 63) / 64;
+    constexpr int ZB = NG * RZ;
+    constexpr int WN = ZB + 2 * L;  // window rows z0-L .. z0+ZB-1+L
+
+    const int lane = threadIdx.x & 63, w = wave_id();
+    const int mzr = mb / J.mnxb, mxb = mb - mzr * J.mnxb;
+    const int mt = 4 * mxb + w;
+    if (mt >= J.mntx) return;
+    const int txi = J.mta + mt;
+    const int x0 = txi * kTX, z0 = mzr * ZB;
+    const StepCtx S = step_ctx(J, step);
+    const uint32_t rowb = S.rowb;
+    const __amdgpu_buffer_rsrc_t rP = buf_rsrc(J.p[S.par], S.nbytes);
+    const __amdgpu_buffer_rsrc_t rO = buf_rsrc(J.p[S.par ^ 1], S.nbytes);
+    const __amdgpu_buffer_rsrc_t rV = buf_rsrc(J.vdt2, S.nbytes);
+    const int j = x0 + lane;
+    const uint32_t vcol = (uint32_t)(j + kHX) * 4u;
+    const uint32_t srow0 = (uint32_t)(z0 + kHZ) * rowb;
+    int h_lds[HPL];
+    uint32_t h_off[HPL];
+    halo_slots<N, RZ>(lane, h_lds, h_off, x0, J.pitch);
+
+    float win[WN], po[ZB], v2[ZB], hal[NG][HPL];
+    float *sp = lds_p + w * (RZ * PW);
+    float wmax = 0.f;
+    // group q's loads: its RZ new window rows (group 0: all RZ + 2L), p_old,
+    // v^2dt^2 and x halo
+    auto issue = [&](int q) {
+        const uint32_t sn = srow0 + q * RZ * rowb;
+        if (q == 0) {
+#pragma unroll
+            for (int r = 0; r < 2 * L; ++r) win[r] = buf_ld(rP, vcol, srow0 + (r - L) * rowb);
+        }
+#pragma unroll
+        for (int r = 0; r < RZ; ++r) {
+            win[q * RZ + 2 * L + r] = buf_ld(rP, vcol, sn + (r + L) * rowb);
+            po[q * RZ + r] = buf_ld(rO, vcol, sn + r * rowb);
+            v2[q * RZ + r] = buf_ld(rV, vcol, sn + r * rowb);
+        }
+#pragma unroll
+        for (int u = 0; u < HPL; ++u) hal[q][u] = buf_ld(rP, h_off[u], sn);
+    };
+    issue(0);
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        // keep the prefetch one group deep: nothing below moves above this point
+        __builtin_amdgcn_sched_barrier(0);
+        if (g + 1 < NG) issue(g + 1);
+        const int zs = z0 + g * RZ;
+#pragma unroll
+        for (int r = 0; r < RZ; ++r) sp[r * PW + L + lane] = win[g * RZ + L + r];
+#pragma unroll
+        for (int u = 0; u < HPL; ++u)
+            if (h_lds[u] >= 0) sp[h_lds[u]] = hal[g][u];
+        wave_lds_sync();
+        float lap[RZ], s[RZ], pc[RZ], pog[RZ], v2g[RZ];
+        interior_lap<N, RZ>(J, win, g * RZ, sp, lane, lap);
+        tile_sources<RZ>(J, S.t, zs, txi, j, true, s);
+#pragma unroll
+        for (int r = 0; r < RZ; ++r) {
+            pc[r] = win[g * RZ + L + r];
+            pog[r] = po[g * RZ + r];
+            v2g[r] = v2[g * RZ + r];
+        }
+        tile_update<RZ>(J, S, rO, vcol, srow0 + g * RZ * rowb, zs, j, true, pc, pog, v2g, lap, s,
+                        wmax);
+    }
+    if (S.pack_now && mb == 0 && w == 0 && lane == 0) J.ck_scale[S.c - 1] = S.scale;
+    ck_max_commit(J, S, wmax, mb * 4 + w, lane);
+}
+
+// Block entry: order entries >= 0 are 4-tile blocks (tile path), entries < 0
+// interior march blocks (-1 - index).
+template <int N, int RZ, int NG>
+__device__ __forceinline__ void cpml2d_block(const Job2D &J, int tile, int step) {
+    __shared__ float s_p[4 * RZ * (kTX + 2 * (2 * N - 1))];
+    __shared__ float s_a[4 * RZ * (kTX + 2 * N - 1)];
+    if constexpr (NG > 0) {
+        if (tile < 0) {
+            cpml2d_march<N, RZ, NG>(J, -1 - tile, step, s_p);
+            return;
+        }
+    }
+    cpml2d_tile<N, RZ>(J, tile, step, s_p, s_a);
+}
+
+template <int N, int RZ, int NG>
 __global__ void __launch_bounds__(256) cpml2d_single(Job2D J, const int2 *__restrict__ order,
                                                      int step) {
-    cpml2d_tile<N, RZ>(J, order[blockIdx.x].y, step);
+    cpml2d_block<N, RZ, NG>(J, order[blockIdx.x].y, step);
 }
 
-template <int N, int RZ>
+template <int N, int RZ, int NG>
 __global__ void __launch_bounds__(256) cpml2d_batch(const Job2D *__restrict__ jobs,
                                                     const int2 *__restrict__ order, int step) {
     const int2 jt = order[blockIdx.x];
-    cpml2d_tile<N, RZ>(jobs[jt.x], jt.y, step);
+    cpml2d_block<N, RZ, NG>(jobs[jt.x], jt.y, step);
 }
 
 // Pack a staged FP32 checkpoint into 16 bits after the job's last step.
@@ -398,13 +592,39 @@ static void composite_weights(int N, const double *c, double *w) {
 }
 
 Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB_, double dz_,
-                       double dx_, double dt_, int rz_)
+                       double dx_, double dt_, int rz_, int ng_)
     : nz(nz_), nx(nx_), N(N_), nB(nB_), dz(dz_), dx(dx_), dt(dt_) {
     rz = 4;
     if (rz_ == 4 || rz_ == 8) rz = rz_;
     else if (const char *e = std::getenv("SSSI_RZ")) {
         const int r = std::atoi(e);
         if (r == 4 || r == 8) rz = r;
+    }
+    ng = ng_ >= 0 ? ng_ : march_groups(rz);
+    if (!(ng == 0 || ng == 4 || (ng == 8 && rz == 4))) fail(SSSI_EINVAL, "march groups %d", ng);
+    if (ng > 0) {
+        // column tiles with every lane inside the model and no x strip within
+        // stencil reach (the tile path's interior test); a contiguous range
+        int ta = -1, tb = -1;
+        for (int t = 0; t < ntx(); ++t) {
+            const int64_t x0 = (int64_t)t * kTX;
+            bool ok = x0 + kTX <= nx;
+            if (nB > 0) ok = ok && x0 - N >= nB && x0 + kTX + N - 1 <= nx - nB;
+            if (ok) {
+                if (ta < 0) ta = t;
+                tb = t;
+            }
+        }
+        // rows with no z strip within reach, in whole tile blocks and march bands
+        const int64_t zlim = nB > 0 ? nz - nB - N + 1 : nz;
+        const int64_t unit = (int64_t)rz * std::lcm(4, ng);
+        mz_rows = zlim >= unit ? (int)((zlim / unit) * unit) : 0;
+        if (ta >= 0 && mz_rows > 0) {
+            mta = ta;
+            mntx = tb - ta + 1;
+        } else {
+            mz_rows = 0;
+        }
     }
     pitch = round_up(kHX + round_up(nx, kTX) + 32, 64);
     rows = kHZ + round_up(nz, kTZmax) + kHZbot;
@@ -465,6 +685,23 @@ bool Cpml2dGrid::block_is_strip(int tile) const {
     return zs || xs;
 }
 
+bool Cpml2dGrid::block_in_march(int tile) const {
+    if (mntx == 0) return false;
+    const int tzi = tile / ntx(), txi = tile % ntx();
+    return txi >= mta && txi < mta + mntx && (int64_t)(tzi + 1) * tz() <= mz_rows;
+}
+
+static int env_int(const char *name, int dflt) {
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+
+int march_groups(int rz) {
+    int ng = env_int("SSSI_NG", 0);
+    if (rz == 8 && ng > 4) ng = 4;
+    return (ng == 4 || ng == 8) ? ng : 0;
+}
+
 void Cpml2dGrid::zero_state(hipStream_t s) {
     for (auto *b : {&p[0], &p[1], &zphi[0], &zphi[1], &zpsi, &xphi[0], &xphi[1], &xpsi})
         if (b->p) SSSI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), s));
@@ -475,6 +712,7 @@ Job2D Cpml2dGrid::job() const {
     std::memset(&J, 0, sizeof(J));
     J.nz = (int)nz; J.nx = (int)nx; J.pitch = (int)pitch; J.rows = (int)rows;
     J.ntx = ntx(); J.nwz = nwz(); J.nB = nB;
+    J.mz_rows = mz_rows; J.mta = mta; J.mntx = mntx; J.mnxb = march_nxb();
     for (int k = 0; k < 8; ++k) { J.cz[k] = cz[k]; J.cx[k] = cx[k]; }
     J.w0 = w0;
     J.w0lo = w0lo;
@@ -543,54 +781,70 @@ void DevSources::set(const Cpml2dGrid &g, int n_, const int *iz, const int *ix,
     SSSI_HIP(hipMemcpy(amp.p, ha.data(), amp.bytes(), hipMemcpyHostToDevice));
 }
 
-void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector<int2> &order) {
-    std::vector<int2> strip, inner;
-    for (int g = 0; g < (int)grids.size(); ++g)
-        for (int t = 0; t < grids[g]->nblocks(); ++t)
-            (grids[g]->block_is_strip(t) ? strip : inner).push_back(make_int2(g, t));
-    order = strip;
-    const size_t ni = inner.size(), total = strip.size() + inner.size();
+// Append `list` so that the k-th appended entry lands on XCD (base + k) % 8
+// with each XCD taking a contiguous band of the list.
+static void append_banded(std::vector<int2> &order, const std::vector<int2> &list) {
+    const size_t n = list.size(), base = order.size();
     size_t next[8], end[8];
     for (int x = 0; x < 8; ++x) {
-        next[x] = ni * x / 8;
-        end[x] = ni * (x + 1) / 8;
+        next[x] = n * x / 8;
+        end[x] = n * (x + 1) / 8;
     }
-    for (size_t b = order.size(); b < total; ++b) {
+    for (size_t b = base; b < base + n; ++b) {
         int x = (int)(b % 8);
         if (next[x] == end[x])
             for (int y = 0; y < 8; ++y)
                 if (next[y] < end[y]) { x = y; break; }
-        order.push_back(inner[next[x]++]);
+        order.push_back(list[next[x]++]);
     }
 }
 
-#define SSSI_NRZ_SWITCH(N, RZ, CALL)                                                       \
-    switch ((N) * 16 + (RZ)) {                                                            \
-        case 1 * 16 + 4: CALL(1, 4); break;  case 1 * 16 + 8: CALL(1, 8); break;            \
-        case 2 * 16 + 4: CALL(2, 4); break;  case 2 * 16 + 8: CALL(2, 8); break;            \
-        case 3 * 16 + 4: CALL(3, 4); break;  case 3 * 16 + 8: CALL(3, 8); break;            \
-        case 4 * 16 + 4: CALL(4, 4); break;  case 4 * 16 + 8: CALL(4, 8); break;            \
-        case 5 * 16 + 4: CALL(5, 4); break;  case 5 * 16 + 8: CALL(5, 8); break;            \
-        case 6 * 16 + 4: CALL(6, 4); break;  case 6 * 16 + 8: CALL(6, 8); break;            \
-        case 7 * 16 + 4: CALL(7, 4); break;  case 7 * 16 + 8: CALL(7, 8); break;            \
-        case 8 * 16 + 4: CALL(8, 4); break;  case 8 * 16 + 8: CALL(8, 8); break;            \
-        default: fail(SSSI_EINVAL, "no 2-D kernel for nDiffOrder=%d rz=%d", (N), (RZ));    \
+void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector<int2> &order) {
+    std::vector<int2> march, strip, inner;
+    for (int g = 0; g < (int)grids.size(); ++g) {
+        const Cpml2dGrid &G = *grids[g];
+        for (int b = 0; b < G.n_march_blocks(); ++b) march.push_back(make_int2(g, -1 - b));
+        for (int t = 0; t < G.nblocks(); ++t) {
+            if (G.block_in_march(t)) continue;
+            (G.block_is_strip(t) ? strip : inner).push_back(make_int2(g, t));
+        }
+    }
+    order.clear();
+    append_banded(order, march);
+    order.insert(order.end(), strip.begin(), strip.end());
+    append_banded(order, inner);
+}
+
+// Instantiated layouts: (rz, march groups); groups 0 is the tile path alone.
+#define SSSI_NRZ_CASES(NN, CALL)                                                          \
+    case NN * 100 + 4 * 10 + 0: CALL(NN, 4, 0); break;                                    \
+    case NN * 100 + 4 * 10 + 4: CALL(NN, 4, 4); break;                                    \
+    case NN * 100 + 4 * 10 + 8: CALL(NN, 4, 8); break;                                    \
+    case NN * 100 + 8 * 10 + 0: CALL(NN, 8, 0); break;                                    \
+    case NN * 100 + 8 * 10 + 4: CALL(NN, 8, 4); break;
+#define SSSI_NRZ_SWITCH(N, RZ, NG, CALL)                                                   \
+    switch ((N) * 100 + (RZ) * 10 + (NG)) {                                               \
+        SSSI_NRZ_CASES(1, CALL) SSSI_NRZ_CASES(2, CALL) SSSI_NRZ_CASES(3, CALL)           \
+        SSSI_NRZ_CASES(4, CALL) SSSI_NRZ_CASES(5, CALL) SSSI_NRZ_CASES(6, CALL)           \
+        SSSI_NRZ_CASES(7, CALL) SSSI_NRZ_CASES(8, CALL)                                   \
+        default: fail(SSSI_EINVAL, "no 2-D kernel for nDiffOrder=%d rz=%d groups=%d", (N), \
+                      (RZ), (NG));                                                        \
     }
 
 void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
                           int step, hipStream_t s) {
-#define SSSI_L1(NN, R)                                                                    \
-    hipLaunchKernelGGL((cpml2d_single<NN, R>), dim3(nblk), dim3(256), 0, s, job, order, step)
-    SSSI_NRZ_SWITCH(g.N, g.rz, SSSI_L1)
+#define SSSI_L1(NN, R, G)                                                                 \
+    hipLaunchKernelGGL((cpml2d_single<NN, R, G>), dim3(nblk), dim3(256), 0, s, job, order, step)
+    SSSI_NRZ_SWITCH(g.N, g.rz, g.ng, SSSI_L1)
 #undef SSSI_L1
     SSSI_HIP(hipGetLastError());
 }
 
-void launch_cpml2d_batch(int N, int rz, const Job2D *jobs, const int2 *order, int nblk, int step,
-                         hipStream_t s) {
-#define SSSI_LB(NN, R)                                                                    \
-    hipLaunchKernelGGL((cpml2d_batch<NN, R>), dim3(nblk), dim3(256), 0, s, jobs, order, step)
-    SSSI_NRZ_SWITCH(N, rz, SSSI_LB)
+void launch_cpml2d_batch(int N, int rz, int ng, const Job2D *jobs, const int2 *order, int nblk,
+                         int step, hipStream_t s) {
+#define SSSI_LB(NN, R, G)                                                                 \
+    hipLaunchKernelGGL((cpml2d_batch<NN, R, G>), dim3(nblk), dim3(256), 0, s, jobs, order, step)
+    SSSI_NRZ_SWITCH(N, rz, ng, SSSI_LB)
 #undef SSSI_LB
     SSSI_HIP(hipGetLastError());
 }

@@ -206,6 +206,10 @@ class Rtm2D {
     // --- many shots per launch
     void run_batch_phase(std::vector<Cpml2dGrid *> grids, const std::vector<Job2D> &jobs) {
         std::vector<const Cpml2dGrid *> cg(grids.begin(), grids.end());
+        for (const Cpml2dGrid *g : cg)
+            if (g->rz != grids[0]->rz || g->ng != grids[0]->ng)
+                fail(SSSI_ESTATE, "batched shots need one tile layout (rz %d/%d, groups %d/%d)",
+                     g->rz, grids[0]->rz, g->ng, grids[0]->ng);
         std::vector<int2> ord;
         build_block_order(cg, ord);
         DevBuf<int2> o(ord.size());
@@ -213,7 +217,8 @@ class Rtm2D {
         SSSI_HIP(hipMemcpyAsync(o.p, ord.data(), o.bytes(), hipMemcpyHostToDevice, stream));
         SSSI_HIP(hipMemcpyAsync(dj.p, jobs.data(), dj.bytes(), hipMemcpyHostToDevice, stream));
         for (int64_t q = 0; q < nt; ++q) {
-            launch_cpml2d_batch(N, grids[0]->rz, dj.p, o.p, (int)ord.size(), (int)q, stream);
+            launch_cpml2d_batch(N, grids[0]->rz, grids[0]->ng, dj.p, o.p, (int)ord.size(),
+                                (int)q, stream);
             launches++;
         }
         SSSI_HIP(hipStreamSynchronize(stream));

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -98,7 +99,7 @@ class Server {
     std::string fatal;
 
     struct Group {
-        int N, rz;
+        int N, rz, ng;
         std::vector<ServeJob *> members;
         DevBuf<Job2D> desc;
         DevBuf<int2> order;
@@ -363,12 +364,13 @@ class Server {
     void rebuild_groups() {
         for (auto &g : groups) retired.push_back(std::move(g));
         groups.clear();
-        std::map<std::pair<int, int>, Group *> by;
+        std::map<std::tuple<int, int, int>, Group *> by;
         for (ServeJob *j : active) {
             if (j->kind == kFwd3d || j->state != kActive) continue;
-            auto key = std::make_pair(j->g->N, j->g->rz);
+            auto key = std::make_tuple(j->g->N, j->g->rz, j->g->ng);
             if (!by.count(key)) {
-                groups.emplace_back(new Group{key.first, key.second, {}, {}, {}, 0});
+                groups.emplace_back(new Group{std::get<0>(key), std::get<1>(key),
+                                              std::get<2>(key), {}, {}, {}, 0});
                 by[key] = groups.back().get();
             }
             by[key]->members.push_back(j);
@@ -399,7 +401,8 @@ class Server {
     void step_all() {
         if (groups_dirty) rebuild_groups();
         for (auto &g : groups) {
-            launch_cpml2d_batch(g->N, g->rz, g->desc.p, g->order.p, g->nblk, (int)tick, stream);
+            launch_cpml2d_batch(g->N, g->rz, g->ng, g->desc.p, g->order.p, g->nblk,
+                                (int)tick, stream);
             launches++;
             for (ServeJob *j : g->members)
                 cell_updates += (double)j->nz * j->nx;
