@@ -91,7 +91,9 @@ def bench_3d(args):
     wl = workload_3d()
     V = wl["V"]
     nt = wl["nt"] if args.nt is None else args.nt
+    sssi_amd.set_autotune(not args.no_tune)
     P = sssi_amd.Propagator(V, wl["N"], wl["nB"], wl["d"], wl["d"], wl["dt"], dy=wl["d"])
+    ty, zc = P.layout()
     iz, ix, iy = wl["src"]
     P.set_points([iz], [ix], wl["wavelet"], src_iy=[iy])
     bytes_step, cells = P.alg_bytes_per_step()
@@ -118,7 +120,8 @@ def bench_3d(args):
         "data": "synthetic (seeded 8-layer cake with a dipping interface, Ricker source)",
         "config": {"workload": f"3D CPML forward, {V.shape[0]}x{V.shape[1]}x{V.shape[2]} ext "
                                f"grid, nDiffOrder=4, nt={nt}, dt=1ms, 1 shot",
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU",
+                   "layout": {"block_y": ty, "z_chunk": zc, "autotuned": not args.no_tune}},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "cpml3d_step", "alg_bytes_per_launch": bytes_step,
@@ -291,6 +294,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="2d/3d: default kernel layout instead of the autotuned one")
     ap.add_argument("--workload", default="2d", choices=["2d", "3d", "rtm", "serve", "copy"],
                     help="2d: the headline 2-D forward line (default); 3d / rtm: other workloads")
     ap.add_argument("--nt", type=int, default=None, help="3d: time steps (default 2000)")
@@ -321,7 +326,9 @@ def main():
     wl = workload()
     V = wl["V"]
     nt = wl["nt"]
+    sssi_amd.set_autotune(not args.no_tune)
     P = sssi_amd.Propagator(V, wl["N"], wl["nB"], wl["dz"], wl["dx"], wl["dt"])
+    rz, ng = P.layout()
     P.set_points(wl["src_iz"], wl["src_ix"], wl["wavelet"])
     bytes_step, cells = P.alg_bytes_per_step()
 
@@ -365,11 +372,13 @@ def main():
         "config": {"workload": "2D CPML forward, 1040x3080 ext grid (1000x3000 + nB=40), "
                                "nDiffOrder=5, nt=4000, dt=0.5ms, 1 shot, 3080 receivers",
                    "nz": int(V.shape[0]), "nx": int(V.shape[1]), "nt": nt,
-                   "nDiffOrder": wl["N"], "nBoundary": wl["nB"], "parallelism": "1 GPU"},
+                   "nDiffOrder": wl["N"], "nBoundary": wl["nB"], "parallelism": "1 GPU",
+                   "layout": {"tile_rows": rz, "march_groups": ng, "autotuned": not args.no_tune,
+                              "graph": os.environ.get("SSSI_GRAPH", "0") != "0"}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(),
-                     "kernel": "cpml2d_step", "alg_bytes_per_launch": bytes_step,
+                     "kernel": "cpml2d_single", "alg_bytes_per_launch": bytes_step,
                      "avg_launch_us": round(avg_launch_s * 1e6, 3), "launches": launches},
     }
     if not args.no_cpu:

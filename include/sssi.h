@@ -48,6 +48,15 @@ int sssi_version(void);
 int sssi_device_count(int *count);
 /* Select the HIP device used by subsequently created handles and calls. */
 int sssi_set_device(int device);
+/* Layout autotuner (no counterpart in the reference): with on = 1, every
+ * propagation handle created afterwards times the instantiated kernel layouts
+ * on its own grid (2-D: tile height and interior march groups; 3-D: block y
+ * extent and z chunk) and keeps the fastest, cached per problem shape (and
+ * appended to the file named by SSSI_TUNE_CACHE, if set).  Off by default
+ * (SSSI_AUTOTUNE=1 turns it on); results of different layouts agree with the
+ * oracle to the same tolerance, and may differ from each other in the last
+ * bits near damping-strip edges. */
+int sssi_set_autotune(int on);
 
 /* ---------------------------------------------------------------------
  * Drop-in entry points: same arguments, order and outputs as the MEX
@@ -154,6 +163,11 @@ int sssi_prop_reset(sssi_prop *p);
  * (reverse = 1; surface data, t = nt-1..0).  Receivers (top row) are
  * recorded on device when record != 0.  Asynchronous. */
 int sssi_prop_run(sssi_prop *p, int64_t nt, int reverse, int record);
+/* Continue a propagation: count steps from time index t_first (forward:
+ * t_first .. t_first+count-1; reverse: t_first down to t_first-count+1).
+ * A run of nt steps equals consecutive calls covering 0..nt-1 (forward) or
+ * nt-1..0 (reverse), bit for bit.  Asynchronous. */
+int sssi_prop_run_steps(sssi_prop *p, int64_t t_first, int64_t count, int reverse, int record);
 /* Same as sssi_prop_run, timed with HIP events on the handle's stream;
  * returns elapsed milliseconds of the whole launch sequence and the
  * number of kernel launches it made.  Synchronous. */
@@ -170,6 +184,10 @@ int sssi_prop_get_model(sssi_prop *p, double *model);
 /* Whole-model statistics of the grid (cells, strip cells) for roofline
  * accounting: algorithmic bytes per step of the propagation kernel. */
 int sssi_prop_alg_bytes_per_step(sssi_prop *p, double *bytes, double *cells);
+
+/* Kernel layout in use: 2-D (a, b) = (tile height, interior march groups),
+ * 3-D (a, b) = (block y extent, z chunk); 0, 0 for a sponge handle. */
+int sssi_prop_layout(sssi_prop *p, int *a, int *b);
 
 /* ---------------------------------------------------------------------
  * Reverse-time migration, 2-D CPML (the imaging loop of

@@ -18,8 +18,9 @@ Errors raise :class:`SssiError` carrying the ABI status and message.
 The compute runs on the GPU through libsssi; there is no CPU fallback.  When
 the library is missing or no device is visible, calls raise.
 """
-from ._lib import (SssiError, lib, device_count, fwd_cpml_2d_points, rvs_cpml_2d_ex,
-                   fwd_cpml_3d_points, fwd_sponge_2d_points, rvs_sponge_2d_ex, Propagator)
+from ._lib import (SssiError, lib, device_count, set_autotune, fwd_cpml_2d_points,
+                   rvs_cpml_2d_ex, fwd_cpml_3d_points, fwd_sponge_2d_points, rvs_sponge_2d_ex,
+                   Propagator)
 from .api import (fwdTimeCpmlFor2dAw, rvsTimeCpmlFor2dAw, fwdTimeCpmlFor3dAw,
                   fwdTimeSpongeFor2dAw, rvsTimeSpongeFor2dAw)
 from . import synth

@@ -24,6 +24,7 @@ SIGNATURES = {
     "sssi_version": [],
     "sssi_device_count": [_I],
     "sssi_set_device": [_int],
+    "sssi_set_autotune": [_int],
     "sssi_fwdTimeCpmlFor2dAw": [_D, _i64, _i64, _D, _i64, _int, _int, _dbl, _dbl, _dbl, _D, _D],
     "sssi_rvsTimeCpmlFor2dAw": [_D, _i64, _i64, _D, _i64, _int, _int, _dbl, _dbl, _dbl, _D, _D],
     "sssi_fwdTimeCpmlFor3dAw": [_D, _i64, _i64, _i64, _D, _i64, _int, _int, _dbl, _dbl, _dbl,
@@ -47,12 +48,14 @@ SIGNATURES = {
     "sssi_prop_set_surface": [_vp, _D, _i64],
     "sssi_prop_reset": [_vp],
     "sssi_prop_run": [_vp, _i64, _int, _int],
+    "sssi_prop_run_steps": [_vp, _i64, _i64, _int, _int],
     "sssi_prop_run_timed": [_vp, _i64, _int, _int, _D, ctypes.POINTER(_i64)],
     "sssi_prop_sync": [_vp],
     "sssi_prop_get_data": [_vp, _D, _i64],
     "sssi_prop_get_field": [_vp, _D],
     "sssi_prop_get_model": [_vp, _D],
     "sssi_prop_alg_bytes_per_step": [_vp, _D, _D],
+    "sssi_prop_layout": [_vp, _I, _I],
     "sssi_rtm2d_create": [_D, _D, _i64, _i64, _int, _int, _dbl, _dbl, _dbl, _i64, _int, _int, _dbl,
                           ctypes.POINTER(_vp)],
     "sssi_rtm2d_add_shot": [_vp, _i64, _i64, _int, _D, _I],
@@ -114,6 +117,12 @@ def device_count():
     n = ctypes.c_int(0)
     _call("sssi_device_count", ctypes.byref(n))
     return n.value
+
+
+def set_autotune(on=True):
+    """Handles created afterwards time the kernel layouts on their own grid and
+    keep the fastest (cached per problem shape); see sssi_set_autotune."""
+    _call("sssi_set_autotune", 1 if on else 0)
 
 
 def _dp(a):
@@ -256,6 +265,11 @@ class Propagator:
     def run(self, nt, reverse=False, record=True):
         _call("sssi_prop_run", self._h, int(nt), int(bool(reverse)), int(bool(record)))
 
+    def run_steps(self, t_first, count, reverse=False, record=True):
+        """Continue from time index t_first (see sssi_prop_run_steps)."""
+        _call("sssi_prop_run_steps", self._h, int(t_first), int(count), int(bool(reverse)),
+              int(bool(record)))
+
     def run_timed(self, nt, reverse=False, record=True):
         ms = ctypes.c_double(0)
         n = ctypes.c_int64(0)
@@ -290,3 +304,10 @@ class Propagator:
         c = ctypes.c_double(0)
         _call("sssi_prop_alg_bytes_per_step", self._h, ctypes.byref(b), ctypes.byref(c))
         return b.value, c.value
+
+    def layout(self):
+        """2-D: (tile height, interior march groups); 3-D: (block y extent, z chunk)."""
+        a = ctypes.c_int(0)
+        b = ctypes.c_int(0)
+        _call("sssi_prop_layout", self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsssi.so")
 
-SOURCES = ["common.cpp", "abi.cpp", "cpml2d.hip", "prop2d.hip", "prop3d.hip", "rtm.hip", "serve.cpp"]
+SOURCES = ["common.cpp", "tune.cpp", "abi.cpp", "cpml2d.hip", "prop2d.hip", "prop3d.hip", "rtm.hip",
+           "serve.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

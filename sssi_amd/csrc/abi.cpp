@@ -6,6 +6,7 @@
 #include <memory>
 
 #include "prop.h"
+#include "tune.h"
 
 using namespace sssi;
 
@@ -144,6 +145,10 @@ int sssi_device_count(int *count) {
 
 int sssi_set_device(int device) {
     return guarded([&] { SSSI_HIP(hipSetDevice(device)); });
+}
+
+int sssi_set_autotune(int on) {
+    return guarded([&] { set_autotune(on != 0); });
 }
 
 int sssi_fwd_cpml_2d_points(const double *v, int64_t nz, int64_t nx, int64_t nt, int nsrc,
@@ -335,6 +340,16 @@ int sssi_prop_run(sssi_prop *p, int64_t nt, int reverse, int record) {
     });
 }
 
+int sssi_prop_run_steps(sssi_prop *p, int64_t t_first, int64_t count, int reverse, int record) {
+    return guarded([&] {
+        if (count < 1) fail(SSSI_EINVAL, "count must be >= 1");
+        if (t_first < 0 || (reverse && t_first - count + 1 < 0))
+            fail(SSSI_EINVAL, "steps outside t >= 0 (t_first=%lld count=%lld)",
+                 (long long)t_first, (long long)count);
+        impl(p).run(t_first, count, reverse != 0, record != 0, nullptr, nullptr);
+    });
+}
+
 int sssi_prop_run_timed(sssi_prop *p, int64_t nt, int reverse, int record, double *ms,
                         int64_t *launches) {
     return guarded([&] {
@@ -388,6 +403,14 @@ int sssi_prop_alg_bytes_per_step(sssi_prop *p, double *bytes, double *cells) {
         need(bytes, "bytes");
         need(cells, "cells");
         impl(p).alg_bytes(bytes, cells);
+    });
+}
+
+int sssi_prop_layout(sssi_prop *p, int *a, int *b) {
+    return guarded([&] {
+        need(a, "a");
+        need(b, "b");
+        impl(p).layout(a, b);
     });
 }
 

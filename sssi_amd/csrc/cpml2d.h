@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "prop.h"
@@ -74,6 +75,9 @@ class Cpml2dGrid {
     // rz 0 / ng -1: the SSSI_RZ / SSSI_NG environment variables or the defaults
     Cpml2dGrid(const double *v, int64_t nz, int64_t nx, int N, int nB, double dz, double dx,
                double dt, int rz = 0, int ng = -1);
+    // tile height (4 or 8) and interior march groups (0, 4, 8 with rz 4): the
+    // block geometry only; arrays and results at one rz do not depend on ng
+    void set_layout(int rz, int ng);
     int ntx() const { return (int)((nx + kTX - 1) / kTX); }
     int nwz() const { return (int)((nz + rz - 1) / rz); }
     int tz() const { return 4 * rz; }
@@ -112,13 +116,45 @@ int march_groups(int rz);
 // split into 8 contiguous bands (block b is dispatched to XCD b % 8).
 void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector<int2> &order);
 
-// Launch one step of a single job (descriptor by value).
+// Launch one step of a single job (descriptor by value).  With `base`, the
+// kernel's step is step + *base (a launch inside a captured time loop).
 void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
-                          int step, hipStream_t s);
+                          int step, hipStream_t s, const int *base = nullptr);
 // Launch one step of many jobs sharing nDiffOrder, rz and march layout
 // (descriptors in device memory).
 void launch_cpml2d_batch(int N, int rz, int ng, const Job2D *jobs, const int2 *order, int nblk,
-                         int step, hipStream_t s);
+                         int step, hipStream_t s, const int *base = nullptr);
+// hipGraph-captured time loop: kGraphSteps step launches that read their step
+// base from device memory, then a one-thread kernel that advances the base, so
+// one captured graph replays for any run of the same launch configuration.
+// Replays launch the same kernels with the same step indices as direct
+// launches.  Capture is on with SSSI_GRAPH=1 (read at every run); direct
+// back-to-back launches are as fast on this kernel, so it is off by default.
+constexpr int kGraphSteps = 64;
+bool graphs_enabled();
+
+class StepGraph {
+   public:
+    using Launch = std::function<void(int step, const int *base, hipStream_t s)>;
+    StepGraph() = default;
+    StepGraph(const StepGraph &) = delete;
+    StepGraph &operator=(const StepGraph &) = delete;
+    ~StepGraph();
+    // Steps step0 .. step0+count-1.  `key` (compared bytewise) identifies the
+    // launch configuration; a different key re-captures.  clear() (and the
+    // destructor) wait for the last replays' stream, which must still exist.
+    void run(const void *key, size_t key_bytes, int step0, int64_t count, hipStream_t s,
+             const Launch &launch);
+    void clear();
+
+   private:
+    std::vector<char> key_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    hipStream_t stream_ = nullptr;  // stream of the last replays
+    DevBuf<int> base_;
+};
+
 // Pack the staged FP32 checkpoint `c` of a job into 16-bit (after its last step).
 void launch_ck_pack(const Job2D &job, int c, hipStream_t s);
 

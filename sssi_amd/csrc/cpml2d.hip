@@ -549,17 +549,29 @@ __device__ __forceinline__ void cpml2d_block(const Job2D &J, int tile, int step)
     cpml2d_tile<N, RZ>(J, tile, step, s_p, s_a);
 }
 
+// step = the launch's step argument, plus a device-resident base when the
+// launch is replayed from a captured time loop (StepGraph).
 template <int N, int RZ, int NG>
 __global__ void __launch_bounds__(256) cpml2d_single(Job2D J, const int2 *__restrict__ order,
-                                                     int step) {
+                                                     int step, const int *__restrict__ base) {
+    if (base) step += ld_uniform(base);
     cpml2d_block<N, RZ, NG>(J, order[blockIdx.x].y, step);
 }
 
 template <int N, int RZ, int NG>
 __global__ void __launch_bounds__(256) cpml2d_batch(const Job2D *__restrict__ jobs,
-                                                    const int2 *__restrict__ order, int step) {
+                                                    const int2 *__restrict__ order, int step,
+                                                    const int *__restrict__ base) {
+    if (base) step += ld_uniform(base);
     const int2 jt = order[blockIdx.x];
     cpml2d_block<N, RZ, NG>(jobs[jt.x], jt.y, step);
+}
+
+__global__ void step_base_set(int *base, int v) {
+    if (threadIdx.x == 0) *base = v;
+}
+__global__ void step_base_add(int *base, int k) {
+    if (threadIdx.x == 0) *base += k;
 }
 
 // Pack a staged FP32 checkpoint into 16 bits after the job's last step.
@@ -594,38 +606,13 @@ static void composite_weights(int N, const double *c, double *w) {
 Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB_, double dz_,
                        double dx_, double dt_, int rz_, int ng_)
     : nz(nz_), nx(nx_), N(N_), nB(nB_), dz(dz_), dx(dx_), dt(dt_) {
-    rz = 4;
-    if (rz_ == 4 || rz_ == 8) rz = rz_;
+    int r = 4;
+    if (rz_ == 4 || rz_ == 8) r = rz_;
     else if (const char *e = std::getenv("SSSI_RZ")) {
-        const int r = std::atoi(e);
-        if (r == 4 || r == 8) rz = r;
+        const int q = std::atoi(e);
+        if (q == 4 || q == 8) r = q;
     }
-    ng = ng_ >= 0 ? ng_ : march_groups(rz);
-    if (!(ng == 0 || ng == 4 || (ng == 8 && rz == 4))) fail(SSSI_EINVAL, "march groups %d", ng);
-    if (ng > 0) {
-        // column tiles with every lane inside the model and no x strip within
-        // stencil reach (the tile path's interior test); a contiguous range
-        int ta = -1, tb = -1;
-        for (int t = 0; t < ntx(); ++t) {
-            const int64_t x0 = (int64_t)t * kTX;
-            bool ok = x0 + kTX <= nx;
-            if (nB > 0) ok = ok && x0 - N >= nB && x0 + kTX + N - 1 <= nx - nB;
-            if (ok) {
-                if (ta < 0) ta = t;
-                tb = t;
-            }
-        }
-        // rows with no z strip within reach, in whole tile blocks and march bands
-        const int64_t zlim = nB > 0 ? nz - nB - N + 1 : nz;
-        const int64_t unit = (int64_t)rz * std::lcm(4, ng);
-        mz_rows = zlim >= unit ? (int)((zlim / unit) * unit) : 0;
-        if (ta >= 0 && mz_rows > 0) {
-            mta = ta;
-            mntx = tb - ta + 1;
-        } else {
-            mz_rows = 0;
-        }
-    }
+    set_layout(r, ng_ >= 0 ? ng_ : march_groups(r));
     pitch = round_up(kHX + round_up(nx, kTX) + 32, 64);
     rows = kHZ + round_up(nz, kTZmax) + kHZbot;
     const size_t np = (size_t)pitch * rows;
@@ -683,6 +670,37 @@ bool Cpml2dGrid::block_is_strip(int tile) const {
     const bool zs = z0 + tz() + N - 1 > nz - nB;
     const bool xs = x0 - N < nB || x0 + kTX + N - 1 > nx - nB;
     return zs || xs;
+}
+
+void Cpml2dGrid::set_layout(int rz_, int ng_) {
+    if (!(rz_ == 4 || rz_ == 8)) fail(SSSI_EINVAL, "tile height %d (4 or 8)", rz_);
+    if (!(ng_ == 0 || ng_ == 4 || (ng_ == 8 && rz_ == 4)))
+        fail(SSSI_EINVAL, "march groups %d with tile height %d", ng_, rz_);
+    rz = rz_;
+    ng = ng_;
+    mz_rows = mta = mntx = 0;
+    if (ng == 0) return;
+    // column tiles with every lane inside the model and no x strip within
+    // stencil reach (the tile path's interior test); a contiguous range
+    int ta = -1, tb = -1;
+    for (int t = 0; t < ntx(); ++t) {
+        const int64_t x0 = (int64_t)t * kTX;
+        bool ok = x0 + kTX <= nx;
+        if (nB > 0) ok = ok && x0 - N >= nB && x0 + kTX + N - 1 <= nx - nB;
+        if (ok) {
+            if (ta < 0) ta = t;
+            tb = t;
+        }
+    }
+    // rows with no z strip within reach, in whole tile blocks and march bands
+    const int64_t zlim = nB > 0 ? nz - nB - N + 1 : nz;
+    const int64_t unit = (int64_t)rz * std::lcm(4, ng);
+    const int64_t rows_m = zlim >= unit ? (zlim / unit) * unit : 0;
+    if (ta >= 0 && rows_m > 0) {
+        mz_rows = (int)rows_m;
+        mta = ta;
+        mntx = tb - ta + 1;
+    }
 }
 
 bool Cpml2dGrid::block_in_march(int tile) const {
@@ -832,18 +850,20 @@ void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector
     }
 
 void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
-                          int step, hipStream_t s) {
+                          int step, hipStream_t s, const int *base) {
 #define SSSI_L1(NN, R, G)                                                                 \
-    hipLaunchKernelGGL((cpml2d_single<NN, R, G>), dim3(nblk), dim3(256), 0, s, job, order, step)
+    hipLaunchKernelGGL((cpml2d_single<NN, R, G>), dim3(nblk), dim3(256), 0, s, job, order, step, \
+                       base)
     SSSI_NRZ_SWITCH(g.N, g.rz, g.ng, SSSI_L1)
 #undef SSSI_L1
     SSSI_HIP(hipGetLastError());
 }
 
 void launch_cpml2d_batch(int N, int rz, int ng, const Job2D *jobs, const int2 *order, int nblk,
-                         int step, hipStream_t s) {
+                         int step, hipStream_t s, const int *base) {
 #define SSSI_LB(NN, R, G)                                                                 \
-    hipLaunchKernelGGL((cpml2d_batch<NN, R, G>), dim3(nblk), dim3(256), 0, s, jobs, order, step)
+    hipLaunchKernelGGL((cpml2d_batch<NN, R, G>), dim3(nblk), dim3(256), 0, s, jobs, order, step, \
+                       base)
     SSSI_NRZ_SWITCH(N, rz, ng, SSSI_LB)
 #undef SSSI_LB
     SSSI_HIP(hipGetLastError());
@@ -855,6 +875,52 @@ void launch_ck_pack(const Job2D &job, int c, hipStream_t s) {
                        job.ck_stage + (size_t)(c & 1) * n, job.ck16 + (size_t)c * n,
                        job.ck_max + (size_t)c * kCkSlots, job.ck_scale + c, n, job.ck_fmt);
     SSSI_HIP(hipGetLastError());
+}
+
+bool graphs_enabled() { return env_int("SSSI_GRAPH", 0) != 0; }
+
+StepGraph::~StepGraph() { clear(); }
+
+void StepGraph::clear() {
+    if (exec_ && stream_) (void)hipStreamSynchronize(stream_);  // replays still queued
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+    key_.clear();
+}
+
+void StepGraph::run(const void *key, size_t key_bytes, int step0, int64_t count, hipStream_t s,
+                    const Launch &launch) {
+    const int64_t reps = count / kGraphSteps;
+    if (!graphs_enabled() || reps < 2) {
+        for (int64_t q = 0; q < count; ++q) launch((int)(step0 + q), nullptr, s);
+        return;
+    }
+    const char *kb = static_cast<const char *>(key);
+    if (!exec_ || key_.size() != key_bytes || std::memcmp(key_.data(), kb, key_bytes) != 0) {
+        clear();
+        if (!base_.p) base_.alloc(1);
+        SSSI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int q = 0; q < kGraphSteps; ++q) launch(q, base_.p, s);
+            hipLaunchKernelGGL(step_base_add, dim3(1), dim3(64), 0, s, base_.p, kGraphSteps);
+            SSSI_HIP(hipGetLastError());
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SSSI_HIP(hipStreamEndCapture(s, &graph_));
+        SSSI_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        key_.assign(kb, kb + key_bytes);
+    }
+    stream_ = s;
+    hipLaunchKernelGGL(step_base_set, dim3(1), dim3(64), 0, s, base_.p, step0);
+    SSSI_HIP(hipGetLastError());
+    for (int64_t r = 0; r < reps; ++r) SSSI_HIP(hipGraphLaunch(exec_, s));
+    for (int64_t q = reps * kGraphSteps; q < count; ++q) launch((int)(step0 + q), nullptr, s);
 }
 
 }  // namespace sssi

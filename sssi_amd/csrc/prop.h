@@ -86,7 +86,9 @@ class Prop {
         if (rec_nt < nt) rec_nt = nt;
     }
     // Run on a stream owned by someone else (the serving loop).
-    void use_stream(hipStream_t s) {
+    // layout in use: 2-D (tile height, march groups), 3-D (block y extent, z chunk)
+    virtual void layout(int *a, int *b) const = 0;
+    virtual void use_stream(hipStream_t s) {
         if (own_stream && stream) (void)hipStreamDestroy(stream);
         stream = s;
         own_stream = false;

@@ -8,6 +8,7 @@
 #include <numeric>
 
 #include "cpml2d.h"
+#include "tune.h"
 
 namespace sssi {
 
@@ -75,6 +76,7 @@ class Prop2D final : public Prop {
     DevSources srcs;
     DevBuf<int2> order;
     int nblk = 0;
+    StepGraph graph;   // captured time loop (re-captured when the descriptor changes)
     int64_t done = 0;  // steps since reset (parity of the current level)
     // sponge
     int64_t pitch = 0, rows = 0;
@@ -91,11 +93,8 @@ class Prop2D final : public Prop {
         SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         if (abc == SSSI_ABC_CPML) {
             g.reset(new Cpml2dGrid(v, nz, nx, N, nB, dz, dx, dt));
-            std::vector<int2> ord;
-            build_block_order({g.get()}, ord);
-            nblk = (int)ord.size();
-            order.alloc(ord.size());
-            SSSI_HIP(hipMemcpy(order.p, ord.data(), order.bytes(), hipMemcpyHostToDevice));
+            if (autotune_enabled()) tune_layout();
+            build_order();
         } else {
             pitch = round_up(kHX + round_up(nx, kTX) + 32, 64);
             rows = kHZ + round_up(nz, kTZmax) + kHZbot;
@@ -121,7 +120,74 @@ class Prop2D final : public Prop {
     }
 
     ~Prop2D() override {
+        graph.clear();
         if (stream && own_stream) (void)hipStreamDestroy(stream);
+    }
+
+    void build_order() {
+        std::vector<int2> ord;
+        build_block_order({g.get()}, ord);
+        nblk = (int)ord.size();
+        order.alloc(ord.size());
+        SSSI_HIP(hipMemcpy(order.p, ord.data(), order.bytes(), hipMemcpyHostToDevice));
+    }
+
+    // Autotuner: time every instantiated layout (tile height, march groups) on
+    // this grid -- kTuneSteps launches after a short warm-up, HIP events on the
+    // handle's stream -- keep the fastest, and cache it for the shape.
+    void tune_layout() {
+        const TuneKey key{2, nz, nx, 1, N, nB};
+        int a = 0, b = 0;
+        if (tune_lookup(key, a, b)) {
+            g->set_layout(a, b);
+            return;
+        }
+        static const int cand[][2] = {{4, 0}, {4, 4}, {4, 8}, {8, 0}, {8, 4}};
+        constexpr int kWarm = 4, kTuneSteps = 40;
+        hipEvent_t e0, e1;
+        SSSI_HIP(hipEventCreate(&e0));
+        SSSI_HIP(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        int best_a = g->rz, best_b = g->ng;
+        try {
+            for (const auto &c : cand) {
+                g->set_layout(c[0], c[1]);
+                build_order();
+                const Job2D J = g->job();  // no sources, receivers or snapshots
+                for (int q = 0; q < kWarm; ++q)
+                    launch_cpml2d_single(*g, J, order.p, nblk, q, stream);
+                SSSI_HIP(hipEventRecord(e0, stream));
+                for (int q = 0; q < kTuneSteps; ++q)
+                    launch_cpml2d_single(*g, J, order.p, nblk, kWarm + q, stream);
+                SSSI_HIP(hipEventRecord(e1, stream));
+                SSSI_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                SSSI_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best_a = c[0];
+                    best_b = c[1];
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        g->set_layout(best_a, best_b);
+        tune_store(key, best_a, best_b);
+    }
+
+    void layout(int *a, int *b) const override {
+        *a = g ? g->rz : 0;
+        *b = g ? g->ng : 0;
+    }
+
+    void use_stream(hipStream_t s) override {
+        graph.clear();  // its replays ran on the stream being replaced
+        Prop::use_stream(s);
     }
 
     int64_t cells() const override { return nz * nx; }
@@ -251,10 +317,11 @@ class Prop2D final : public Prop {
         if (g) {
             int step0;
             const Job2D J = make_job(t_first, reverse, record, sink, &step0);
-            for (int64_t q = 0; q < count; ++q) {
-                launch_cpml2d_single(*g, J, order.p, nblk, (int)(step0 + q), stream);
-                if (st) st->launches++;
-            }
+            graph.run(&J, sizeof(J), step0, count, stream,
+                      [&](int step, const int *base, hipStream_t s) {
+                          launch_cpml2d_single(*g, J, order.p, nblk, step, s, base);
+                      });
+            if (st) st->launches += count;
             done += count;
             return;
         }

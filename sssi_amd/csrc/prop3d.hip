@@ -22,6 +22,7 @@
 #include <numeric>
 
 #include "prop.h"
+#include "tune.h"
 
 namespace sssi {
 
@@ -430,10 +431,69 @@ class Prop3D final : public Prop {
             ypsi.alloc(yn);
         }
         reset();
+        if (autotune_enabled()) {
+            tune_layout();
+            reset();
+        }
     }
 
     ~Prop3D() override {
         if (stream && own_stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Autotuner: time every (ty, z chunk) candidate on this grid (a short
+    // warm-up, then kTuneSteps steps timed with HIP events), keep the fastest,
+    // cache it for the shape.  Runs before any source is set; the state is
+    // zeroed afterwards.
+    void tune_layout() {
+        const TuneKey key{3, nz, nx, ny, N, nB};
+        int a = 0, b = 0;
+        if (tune_lookup(key, a, b)) {
+            ty = a;
+            zc = b;
+            return;
+        }
+        static const int cty[] = {4, 8, 16};
+        static const int czc[] = {32, 64, 128};
+        constexpr int kWarm = 2, kTuneSteps = 6;
+        hipEvent_t e0, e1;
+        SSSI_HIP(hipEventCreate(&e0));
+        SSSI_HIP(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        int best_ty = ty, best_zc = zc;
+        try {
+            for (int t : cty)
+                for (int z : czc) {
+                    ty = t;
+                    zc = z;
+                    for (int q = 0; q < kWarm; ++q) step(q, false, nullptr);
+                    SSSI_HIP(hipEventRecord(e0, stream));
+                    for (int q = 0; q < kTuneSteps; ++q) step(kWarm + q, false, nullptr);
+                    SSSI_HIP(hipEventRecord(e1, stream));
+                    SSSI_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    SSSI_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best_ty = t;
+                        best_zc = z;
+                    }
+                }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ty = best_ty;
+        zc = best_zc;
+        tune_store(key, ty, zc);
+    }
+
+    void layout(int *a, int *b) const override {
+        *a = ty;
+        *b = zc;
     }
 
     int64_t cells() const override { return nz * nx * ny; }

@@ -196,11 +196,14 @@ class Rtm2D {
         build_block_order({&g}, ord);
         DevBuf<int2> o(ord.size());
         SSSI_HIP(hipMemcpyAsync(o.p, ord.data(), o.bytes(), hipMemcpyHostToDevice, stream));
-        for (int64_t q = 0; q < nt; ++q) {
-            launch_cpml2d_single(g, J, o.p, (int)ord.size(), (int)q, stream);
-            launches++;
+        {
+            StepGraph graph;
+            graph.run(&J, sizeof(J), 0, nt, stream, [&](int step, const int *base, hipStream_t s) {
+                launch_cpml2d_single(g, J, o.p, (int)ord.size(), step, s, base);
+            });
+            launches += nt;
+            SSSI_HIP(hipStreamSynchronize(stream));  // keep `o` alive until the launches ran
         }
-        SSSI_HIP(hipStreamSynchronize(stream));  // keep `o` alive until the launches ran
     }
 
     // --- many shots per launch
@@ -216,12 +219,17 @@ class Rtm2D {
         DevBuf<Job2D> dj(jobs.size());
         SSSI_HIP(hipMemcpyAsync(o.p, ord.data(), o.bytes(), hipMemcpyHostToDevice, stream));
         SSSI_HIP(hipMemcpyAsync(dj.p, jobs.data(), dj.bytes(), hipMemcpyHostToDevice, stream));
-        for (int64_t q = 0; q < nt; ++q) {
-            launch_cpml2d_batch(N, grids[0]->rz, grids[0]->ng, dj.p, o.p, (int)ord.size(),
-                                (int)q, stream);
-            launches++;
+        {
+            const void *key = dj.p;  // one capture per phase
+            StepGraph graph;
+            graph.run(&key, sizeof(key), 0, nt, stream,
+                      [&](int step, const int *base, hipStream_t s) {
+                          launch_cpml2d_batch(N, grids[0]->rz, grids[0]->ng, dj.p, o.p,
+                                              (int)ord.size(), step, s, base);
+                      });
+            launches += nt;
+            SSSI_HIP(hipStreamSynchronize(stream));
         }
-        SSSI_HIP(hipStreamSynchronize(stream));
     }
 
     void precompute_true(bool batched) {
