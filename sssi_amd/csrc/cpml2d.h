@@ -27,6 +27,7 @@ struct Job2D {
     int mz_rows, mta, mntx, mnxb;
     float cz[8], cx[8];           // c_k/dz, c_k/dx (two-stage form)
     float w0, w0lo, wz[16], wx[16];  // composite D-D+ weights / d^2 (w0 + w0lo: centre)
+    float w0z, w0zlo, w0x, w0xlo;    // centre weights of the z / x stencils alone
     float *p[2];                  // wavefield levels (padded); parity picks current
     const float *vdt2;            // v^2 dt^2 (padded)
     float *zphi[2], *zpsi;        // z strip [(i-(nz-nB))*nx + j]
@@ -71,6 +72,7 @@ class Cpml2dGrid {
     DevBuf<float> p[2], vdt2;
     DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1;
     float cz[8] = {0}, cx[8] = {0}, w0 = 0.f, w0lo = 0.f, wz[16] = {0}, wx[16] = {0};
+    float w0z = 0.f, w0zlo = 0.f, w0x = 0.f, w0xlo = 0.f;
 
     // rz 0 / ng -1: the SSSI_RZ / SSSI_NG environment variables or the defaults
     Cpml2dGrid(const double *v, int64_t nz, int64_t nx, int N, int nB, double dz, double dx,

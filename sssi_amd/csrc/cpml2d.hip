@@ -314,130 +314,156 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
     if (!zstrip && !xstrip) {
         interior_lap<N, RZ>(J, col, 0, sp, lane, lap);
     } else {
-        // ---- damping strips: two-stage form with Phi/Psi
-        const float *zphi_in = J.zphi[par];
-        float *zphi_out = J.zphi[par ^ 1];
-        // A_z at half points h = zs-N+q (q < RZ+L): D+p(h) = sum c_k (p(h+1+k) - p(h-k))
-        float Az[RZ + L];
-#pragma unroll
-        for (int q = 0; q < RZ + L; ++q) {
-            float d = 0.f;
-#pragma unroll
-            for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], col[q + N + k] - col[q + N - 1 - k], d);
-            Az[q] = d;
-        }
-        if (zstrip && jin) {
-            // all z-strip memory variables of the tile in one batch of loads;
-            // (b-1) at half point h and at row i = h share one index
-            float phz[RZ + L], bmh[RZ + L], psz[RZ];
+        // ---- damping strips.  Each direction takes the two-stage form with its
+        // memory variables where its strip is within stencil reach, and its
+        // composite stencil otherwise (equal there, where Phi = Psi = 0).
+        // ---- z: two-stage CPML form where a z strip is within reach
+        if (zstrip) {
+            const float *zphi_in = J.zphi[par];
+            float *zphi_out = J.zphi[par ^ 1];
+            // A_z at half points h = zs-N+q (q < RZ+L): D+p(h) = sum c_k (p(h+1+k) - p(h-k))
+            float Az[RZ + L];
 #pragma unroll
             for (int q = 0; q < RZ + L; ++q) {
-                const int h = zs - N + q;
-                const bool in = h >= zb0 && h < nz;
-                const size_t o = in ? (size_t)(h - zb0) * nx + j : 0;
-                phz[q] = in ? zphi_in[o] : 0.f;
-                bmh[q] = in ? J.zbm1[o] : 0.f;
-            }
-#pragma unroll
-            for (int r = 0; r < RZ; ++r) {
-                const int i = zs + r;
-                const bool in = i >= zb0 && i < nz;
-                psz[r] = in ? J.zpsi[(size_t)(i - zb0) * nx + j] : 0.f;
-            }
-#pragma unroll
-            for (int q = 0; q < RZ + L; ++q) {
-                const int h = zs - N + q;
-                if (h >= zb0 && h < nz) {
-                    const float phi = fmaf(bmh[q], phz[q] + Az[q], phz[q]);  // b*phi + (b-1)*D
-                    Az[q] += phi;
-                    if (h >= zs && h < zs + RZ) zphi_out[(size_t)(h - zb0) * nx + j] = phi;
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RZ; ++r) {
                 float d = 0.f;
 #pragma unroll
-                for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
-                const int i = zs + r;
-                if (i >= zb0 && i < nz) {
-                    const float psi = fmaf(bmh[r + N], psz[r] + d, psz[r]);
-                    J.zpsi[(size_t)(i - zb0) * nx + j] = psi;
-                    d += psi;
+                for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], col[q + N + k] - col[q + N - 1 - k], d);
+                Az[q] = d;
+            }
+            if (zstrip && jin) {
+                // all z-strip memory variables of the tile in one batch of loads;
+                // (b-1) at half point h and at row i = h share one index
+                float phz[RZ + L], bmh[RZ + L], psz[RZ];
+#pragma unroll
+                for (int q = 0; q < RZ + L; ++q) {
+                    const int h = zs - N + q;
+                    const bool in = h >= zb0 && h < nz;
+                    const size_t o = in ? (size_t)(h - zb0) * nx + j : 0;
+                    phz[q] = in ? zphi_in[o] : 0.f;
+                    bmh[q] = in ? J.zbm1[o] : 0.f;
                 }
-                lap[r] = d;
+#pragma unroll
+                for (int r = 0; r < RZ; ++r) {
+                    const int i = zs + r;
+                    const bool in = i >= zb0 && i < nz;
+                    psz[r] = in ? J.zpsi[(size_t)(i - zb0) * nx + j] : 0.f;
+                }
+#pragma unroll
+                for (int q = 0; q < RZ + L; ++q) {
+                    const int h = zs - N + q;
+                    if (h >= zb0 && h < nz) {
+                        const float phi = fmaf(bmh[q], phz[q] + Az[q], phz[q]);  // b*phi + (b-1)*D
+                        Az[q] += phi;
+                        if (h >= zs && h < zs + RZ) zphi_out[(size_t)(h - zb0) * nx + j] = phi;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < RZ; ++r) {
+                    float d = 0.f;
+#pragma unroll
+                    for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
+                    const int i = zs + r;
+                    if (i >= zb0 && i < nz) {
+                        const float psi = fmaf(bmh[r + N], psz[r] + d, psz[r]);
+                        J.zpsi[(size_t)(i - zb0) * nx + j] = psi;
+                        d += psi;
+                    }
+                    lap[r] = d;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < RZ; ++r) {
+                    float d = 0.f;
+#pragma unroll
+                    for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
+                    lap[r] = d;
+                }
             }
         } else {
 #pragma unroll
             for (int r = 0; r < RZ; ++r) {
-                float d = 0.f;
+                float acc = J.w0z * col[L + r];
 #pragma unroll
-                for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
-                lap[r] = d;
+                for (int m = 1; m <= L; ++m)
+                    acc = fmaf(J.wz[m], col[L + r + m] + col[L + r - m], acc);
+                lap[r] = fmaf(J.w0zlo, col[L + r], acc);
             }
         }
-        // A_x at half points h = x0-N+q, q < 64+L: lane q, lanes < L also q = 64+lane
-        const float *xphi_in = J.xphi[par];
-        float *xphi_out = J.xphi[par ^ 1];
-        float *sa = lds_a + w * (RZ * AW);
-        const int nb2 = 2 * nB;
-        const bool xown = xstrip && jin && (j < nB || j >= nx - nB);
-        const int xs = j < nB ? j : j - (nx - nb2);
-        // all x-strip memory variables of the tile in one batch of loads
-        float phx[2][RZ], bmx[2][RZ], psx[RZ], bmo[RZ];
+        // ---- x: two-stage CPML form where an x strip is within reach
+        if (xstrip) {
+            // A_x at half points h = x0-N+q, q < 64+L: lane q, lanes < L also q = 64+lane
+            const float *xphi_in = J.xphi[par];
+            float *xphi_out = J.xphi[par ^ 1];
+            float *sa = lds_a + w * (RZ * AW);
+            const int nb2 = 2 * nB;
+            const bool xown = xstrip && jin && (j < nB || j >= nx - nB);
+            const int xs = j < nB ? j : j - (nx - nb2);
+            // all x-strip memory variables of the tile in one batch of loads
+            float phx[2][RZ], bmx[2][RZ], psx[RZ], bmo[RZ];
 #pragma unroll
-        for (int r = 0; r < RZ; ++r) {
-            const int i = zs + r;
+            for (int r = 0; r < RZ; ++r) {
+                const int i = zs + r;
 #pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const int q = lane + half * kTX;
-                const int h = x0 - N + q;
-                const bool in = xstrip && (half == 0 || lane < L) && i < nz && h >= 0 && h < nx &&
-                                (h < nB || h >= nx - nB);
-                const size_t o = in ? (size_t)i * nb2 + (h < nB ? h : h - (nx - nb2)) : 0;
-                phx[half][r] = in ? xphi_in[o] : 0.f;
-                bmx[half][r] = in ? J.xbm1[o] : 0.f;
-            }
-            const bool own = xown && i < nz;
-            const size_t o = own ? (size_t)i * nb2 + xs : 0;
-            psx[r] = own ? J.xpsi[o] : 0.f;
-            bmo[r] = own ? J.xbm1[o] : 0.f;
-        }
-#pragma unroll
-        for (int r = 0; r < RZ; ++r) {
-            const float *row = sp + r * PW;
-            const int i = zs + r;
-#pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const int q = lane + half * kTX;
-                if (half == 1 && lane >= L) break;
-                float d = 0.f;
-#pragma unroll
-                for (int k = 0; k < N; ++k)
-                    d = fmaf(J.cx[k], row[q + N + k] - row[q + N - 1 - k], d);
-                const int h = x0 - N + q;
-                if (xstrip && i < nz && h >= 0 && h < nx && (h < nB || h >= nx - nB)) {
-                    const float phi = fmaf(bmx[half][r], phx[half][r] + d, phx[half][r]);
-                    d += phi;
-                    if (h >= x0 && h < x0 + kTX)
-                        xphi_out[(size_t)i * nb2 + (h < nB ? h : h - (nx - nb2))] = phi;
+                for (int half = 0; half < 2; ++half) {
+                    const int q = lane + half * kTX;
+                    const int h = x0 - N + q;
+                    const bool in = xstrip && (half == 0 || lane < L) && i < nz && h >= 0 && h < nx &&
+                                    (h < nB || h >= nx - nB);
+                    const size_t o = in ? (size_t)i * nb2 + (h < nB ? h : h - (nx - nb2)) : 0;
+                    phx[half][r] = in ? xphi_in[o] : 0.f;
+                    bmx[half][r] = in ? J.xbm1[o] : 0.f;
                 }
-                sa[r * AW + q] = d;
+                const bool own = xown && i < nz;
+                const size_t o = own ? (size_t)i * nb2 + xs : 0;
+                psx[r] = own ? J.xpsi[o] : 0.f;
+                bmo[r] = own ? J.xbm1[o] : 0.f;
             }
-        }
-        wave_lds_sync();
 #pragma unroll
-        for (int r = 0; r < RZ; ++r) {
-            const float *ar = sa + r * AW + lane + N;
-            float d = 0.f;
+            for (int r = 0; r < RZ; ++r) {
+                const float *row = sp + r * PW;
+                const int i = zs + r;
 #pragma unroll
-            for (int k = 0; k < N; ++k) d = fmaf(J.cx[k], ar[k] - ar[-1 - k], d);
-            const int i = zs + r;
-            if (xown && i < nz) {
-                const float psi = fmaf(bmo[r], psx[r] + d, psx[r]);
-                J.xpsi[(size_t)i * nb2 + xs] = psi;
-                d += psi;
+                for (int half = 0; half < 2; ++half) {
+                    const int q = lane + half * kTX;
+                    if (half == 1 && lane >= L) break;
+                    float d = 0.f;
+#pragma unroll
+                    for (int k = 0; k < N; ++k)
+                        d = fmaf(J.cx[k], row[q + N + k] - row[q + N - 1 - k], d);
+                    const int h = x0 - N + q;
+                    if (xstrip && i < nz && h >= 0 && h < nx && (h < nB || h >= nx - nB)) {
+                        const float phi = fmaf(bmx[half][r], phx[half][r] + d, phx[half][r]);
+                        d += phi;
+                        if (h >= x0 && h < x0 + kTX)
+                            xphi_out[(size_t)i * nb2 + (h < nB ? h : h - (nx - nb2))] = phi;
+                    }
+                    sa[r * AW + q] = d;
+                }
             }
-            lap[r] += d;
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) {
+                const float *ar = sa + r * AW + lane + N;
+                float d = 0.f;
+#pragma unroll
+                for (int k = 0; k < N; ++k) d = fmaf(J.cx[k], ar[k] - ar[-1 - k], d);
+                const int i = zs + r;
+                if (xown && i < nz) {
+                    const float psi = fmaf(bmo[r], psx[r] + d, psx[r]);
+                    J.xpsi[(size_t)i * nb2 + xs] = psi;
+                    d += psi;
+                }
+                lap[r] += d;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) {
+                const float *row = sp + r * PW + L + lane;
+                float acc = J.w0x * col[L + r];
+#pragma unroll
+                for (int m = 1; m <= L; ++m) acc = fmaf(J.wx[m], row[m] + row[-m], acc);
+                lap[r] += fmaf(J.w0xlo, col[L + r], acc);
+            }
         }
     }
 
@@ -636,17 +662,25 @@ Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB
     double w[32];
     composite_weights(N, c, w);
     const int L = 2 * N - 1;
-    double side = 0.0;  // exact in double: a sum of 4L floats
+    double side = 0.0, sidez = 0.0, sidex = 0.0;  // exact in double: sums of floats
     for (int m = 1; m <= L; ++m) {
         wz[m] = (float)(w[L + m] / (dz * dz));
         wx[m] = (float)(w[L + m] / (dx * dx));
         side += 2.0 * ((double)wz[m] + (double)wx[m]);
+        sidez += 2.0 * (double)wz[m];
+        sidex += 2.0 * (double)wx[m];
     }
     // The centre weight is chosen so that the FP32 weights sum to zero (the
     // Laplacian of a constant vanishes, as in the two-stage form where
-    // D+ of a constant is exactly 0); it is carried as w0 + w0lo.
+    // D+ of a constant is exactly 0); it is carried as w0 + w0lo.  The
+    // per-direction centres serve strip tiles that use the composite stencil
+    // in one direction only.
     w0 = (float)(-side);
     w0lo = (float)(-side - (double)w0);
+    w0z = (float)(-sidez);
+    w0zlo = (float)(-sidez - (double)w0z);
+    w0x = (float)(-sidex);
+    w0xlo = (float)(-sidex - (double)w0x);
     if (nB > 0) {
         std::vector<float> zb, xb;
         cpml2d_bm1(v, nz, nx, nB, dz, dx, dt, zb, xb);
@@ -734,6 +768,10 @@ Job2D Cpml2dGrid::job() const {
     for (int k = 0; k < 8; ++k) { J.cz[k] = cz[k]; J.cx[k] = cx[k]; }
     J.w0 = w0;
     J.w0lo = w0lo;
+    J.w0z = w0z;
+    J.w0zlo = w0zlo;
+    J.w0x = w0x;
+    J.w0xlo = w0xlo;
     for (int m = 0; m < 16; ++m) { J.wz[m] = wz[m]; J.wx[m] = wx[m]; }
     J.p[0] = p[0].p; J.p[1] = p[1].p;
     J.vdt2 = vdt2.p;

@@ -60,6 +60,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc3(const void *base, ui
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
                                              0x00020000);
 }
+// Scalar load of wave-uniform, launch-invariant data (source tables): waited
+// for with lgkmcnt, so the in-order vmcnt of the prefetched planes is left alone.
+template <class T>
+__device__ __forceinline__ T ld_uniform3(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
 __device__ __forceinline__ float ld3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
@@ -68,7 +74,10 @@ __device__ __forceinline__ void st3(float v, __amdgpu_buffer_rsrc_t r, uint32_t 
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
 }
 
-template <int N, int TY>
+constexpr int kPD3 = 2;  // planes prefetched ahead by the 3-D sweep
+static_assert(2 * kPD3 + 15 <= kHZbot, "bottom padding must hold the sweep's overrun");
+
+template <int N, int TY, int PD = kPD3>
 __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
     constexpr int L = 2 * N - 1;
     constexpr int NT = 64 * TY;
@@ -99,7 +108,9 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
     const uint32_t vcol = ((uint32_t)(k + kHY) * a.pitch + (uint32_t)(j + kHX)) * 4u;
     auto plane = [&](int i) -> uint32_t { return (uint32_t)(i + kHZ) * a.planeb; };
 
-    // halo slots of this thread in the LDS plane tile (fixed for the sweep)
+    // halo slots of this thread in the LDS plane tile (fixed for the sweep);
+    // threads without a slot load a valid cell that is never written to LDS
+    // (an unconditional load keeps the prefetched registers out of branches)
     int h_lds[HPT];
     uint32_t h_voff[HPT];
 #pragma unroll
@@ -153,56 +164,61 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
         aq[q + 1] = d;
     }
 
-    // prefetch registers for plane zbeg
-    float f_front = ld3(rP, vcol, plane(zbeg + L));
-    float f_po = ld3(rO, vcol, plane(zbeg));
-    float f_v2 = ld3(rV, vcol, plane(zbeg));
-    float f_h[HPT];
+    // prefetch ring: planes zbeg .. zbeg+PD-1 (slot d holds plane i with
+    // (i - zbeg) % PD == d); a slot is refilled right after it is consumed, so
+    // each plane's loads have PD plane-iterations to arrive
+    float f_front[PD], f_po[PD], f_v2[PD], f_h[PD][HPT];
 #pragma unroll
-    for (int u = 0; u < HPT; ++u) f_h[u] = h_lds[u] >= 0 ? ld3(rP, h_voff[u], plane(zbeg)) : 0.f;
+    for (int d = 0; d < PD; ++d) {
+        const int ip = min(zbeg + d, nz - 1);  // in bounds; unused past zend
+        f_front[d] = ld3(rP, vcol, plane(ip + L));
+        f_po[d] = ld3(rO, vcol, plane(ip));
+        f_v2[d] = ld3(rV, vcol, plane(ip));
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) f_h[d][u] = ld3(rP, h_voff[u], plane(ip));
+    }
 
     int buf = 0;
-    for (int i = zbeg; i < zend; ++i) {
+    for (int i0 = zbeg; i0 < zend; i0 += PD) {
+#pragma unroll
+    for (int d = 0; d < PD; ++d) {
+        // planes past the chunk end (last group of a chunk) run on the zero
+        // padding below the model: nothing is stored and no strip array is
+        // touched for them, so the sweep stays free of per-plane branches
+        const int i = i0 + d;
+        const bool live = i < zend;
+        {
         // ---- advance the z window to p(i-L .. i+L) and the A_z queue
 #pragma unroll
         for (int r = 0; r < 2 * L; ++r) zq[r] = zq[r + 1];
-        zq[2 * L] = f_front;
+        zq[2 * L] = f_front[d];
 #pragma unroll
         for (int q = 0; q < 2 * N - 1; ++q) aq[q] = aq[q + 1];
         {
             // new half point h = i+N-1: p(i .. i+2N-1) = zq[L .. L+2N-1]
-            float d = 0.f;
+            float dd = 0.f;
 #pragma unroll
-            for (int m = 0; m < N; ++m) d = fmaf(a.cz[m], zq[L + N + m] - zq[L + N - 1 - m], d);
+            for (int m = 0; m < N; ++m) dd = fmaf(a.cz[m], zq[L + N + m] - zq[L + N - 1 - m], dd);
             const int h = i + N - 1;
             if (nB > 0 && own && h >= zb0 && h < nz) {
                 const size_t o = ((size_t)(h - zb0) * ny + k) * nx + j;
                 const float ph = a.zphi_in[o];
-                const float phi = fmaf(a.zbm1[(size_t)(h - zb0) * nx + j], ph + d, ph);
-                d += phi;
+                const float phi = fmaf(a.zbm1[(size_t)(h - zb0) * nx + j], ph + dd, ph);
+                dd += phi;
                 if (h >= zbeg && h < zend) a.zphi_out[o] = phi;
             }
-            aq[2 * N - 1] = d;
+            aq[2 * N - 1] = dd;
         }
-        const float po = f_po, v2 = f_v2;
+        const float po = f_po[d], v2 = f_v2[d];
         float hv[HPT];
 #pragma unroll
-        for (int u = 0; u < HPT; ++u) hv[u] = f_h[u];
-        // ---- prefetch plane i+1
-        if (i + 1 < zend) {
-            f_front = ld3(rP, vcol, plane(i + 1 + L));
-            f_po = ld3(rO, vcol, plane(i + 1));
-            f_v2 = ld3(rV, vcol, plane(i + 1));
-#pragma unroll
-            for (int u = 0; u < HPT; ++u)
-                f_h[u] = h_lds[u] >= 0 ? ld3(rP, h_voff[u], plane(i + 1)) : 0.f;
-        }
+        for (int u = 0; u < HPT; ++u) hv[u] = f_h[d][u];
 
         // ---- P_z = D-A_z + Psi_z
         float pz = 0.f;
 #pragma unroll
         for (int m = 0; m < N; ++m) pz = fmaf(a.cz[m], aq[N + m] - aq[N - 1 - m], pz);
-        if (nB > 0 && own && i >= zb0) {
+        if (nB > 0 && own && live && i >= zb0) {
             const size_t o = ((size_t)(i - zb0) * ny + k) * nx + j;
             const float ps = a.zpsi[o];
             const float psi = fmaf(a.zbm1[(size_t)(i - zb0) * nx + j], ps + pz, ps);
@@ -238,7 +254,7 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
 #pragma unroll
                 for (int m = 0; m < N; ++m) d = fmaf(a.cx[m], row[q + N + m] - row[q + N - 1 - m], d);
                 const int hx = x0 - N + q;
-                if (xstrip && k < ny && hx >= 0 && hx < nx && (hx < nB || hx >= nx - nB)) {
+                if (xstrip && live && k < ny && hx >= 0 && hx < nx && (hx < nB || hx >= nx - nB)) {
                     const int xs = hx < nB ? hx : hx - (nx - nb2);
                     const size_t o = ((size_t)i * ny + k) * nb2 + xs;
                     const float ph = a.xphi_in[o];
@@ -256,7 +272,7 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
                 for (int m = 0; m < N; ++m)
                     d = fmaf(a.cy[m], cl[(q + N + m) * PX] - cl[(q + N - 1 - m) * PX], d);
                 const int hy = y0 - N + q;
-                if (ystrip && j < nx && hy >= 0 && hy < ny && (hy < nB || hy >= ny - nB)) {
+                if (ystrip && live && j < nx && hy >= 0 && hy < ny && (hy < nB || hy >= ny - nB)) {
                     const int ys = hy < nB ? hy : hy - (ny - nb2);
                     const size_t o = ((size_t)i * nb2 + ys) * nx + j;
                     const float ph = a.yphi_in[o];
@@ -274,7 +290,7 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
             const float *ay = say + (ty + N) * kTX + lane;
 #pragma unroll
             for (int m = 0; m < N; ++m) py = fmaf(a.cy[m], ay[m * kTX] - ay[(-1 - m) * kTX], py);
-            if (xstrip && own && (j < nB || j >= nx - nB)) {
+            if (xstrip && own && live && (j < nB || j >= nx - nB)) {
                 const int xs = j < nB ? j : j - (nx - nb2);
                 const size_t o = ((size_t)i * ny + k) * nb2 + xs;
                 const float ps = a.xpsi[o];
@@ -282,7 +298,7 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
                 a.xpsi[o] = psi;
                 px += psi;
             }
-            if (ystrip && own && (k < nB || k >= ny - nB)) {
+            if (ystrip && own && live && (k < nB || k >= ny - nB)) {
                 const int ys = k < nB ? k : k - (ny - nb2);
                 const size_t o = ((size_t)i * nb2 + ys) * nx + j;
                 const float ps = a.ypsi[o];
@@ -294,12 +310,15 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
         }
 
         // ---- update
-        if (own) {
+        if (own && live) {
             float s = 0.f;
             if (a.nsrc > 0) {
                 const int cell = (i * ny + k) * nx + j;
-                for (int q = 0; q < a.nsrc; ++q)
-                    if (a.src_cell[q] == cell) s += a.src_amp[q];
+                for (int q = 0; q < a.nsrc; ++q) {
+                    const int c = ld_uniform3(a.src_cell + q);
+                    const float amp = ld_uniform3(a.src_amp + q);
+                    if (c == cell) s += amp;
+                }
             }
             const float pc = zq[L];
             const float pn = pc + fmaf(v2, (pz + pxy) + s, pc - po);
@@ -307,7 +326,18 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
             if (a.rec && i == 0) a.rec[(size_t)k * nx + j] = pn;
             if (a.snap) a.snap[((size_t)i * ny + k) * nx + j] = pn;
         }
+        // ---- refill the slot with plane i+PD once its old contents are used
+        // (the loads then reuse the slot's registers and stay in flight across
+        // the loop back-edge); past the chunk they read the bottom padding
+        // (2*PD + L <= kHZbot)
+        f_front[d] = ld3(rP, vcol, plane(i + PD + L));
+        f_po[d] = ld3(rO, vcol, plane(i + PD));
+        f_v2[d] = ld3(rV, vcol, plane(i + PD));
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) f_h[d][u] = ld3(rP, h_voff[u], plane(i + PD));
         buf ^= 1;
+        }
+    }
     }
 }
 
