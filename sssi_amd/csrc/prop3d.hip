@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <vector>
 
 #include "prop.h"
 #include "tune.h"
@@ -37,6 +38,7 @@ struct Cpml3dArgs {
     uint32_t rowb, planeb, nbytes;
     float cz[8], cx[8], cy[8];
     float w0, w0lo, wx[16], wy[16];  // composite x/y weights / d^2 (w0 + w0lo: centre)
+    float w0x, w0xlo, w0y, w0ylo;    // centres of the x / y stencils alone
     // z strip: half points / rows i in [nz-nB, nz): [(i-(nz-nB))*ny + k]*nx + j
     const float *zphi_in;
     float *zphi_out, *zpsi;
@@ -49,6 +51,7 @@ struct Cpml3dArgs {
     const float *yphi_in;
     float *yphi_out, *ypsi;
     const float *ybm1;  // [ys*nx + j]            (replicated over z)
+    const int *order;     // launch order of the blocks (strip blocks first) or null
     const int *src_cell;  // (i*ny + k)*nx + j, sorted
     const float *src_amp;
     int nsrc;
@@ -87,12 +90,13 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
     constexpr int HPT = (HN + NT - 1) / NT;
     constexpr int AXW = kTX + L;  // A_x half points x0-N .. x0+63+N-1 per row
     constexpr int AYH = TY + L;   // A_y half points y0-N .. y0+TY-1+N-1 per column
+    constexpr int QY = (AYH + TY - 1) / TY;  // A_y half points per thread
     __shared__ float sp[2][PX * PY];
     __shared__ float sax[TY * AXW];
     __shared__ float say[AYH * kTX];
 
     const int tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
-    int b = blockIdx.x;
+    int b = a.order ? ld_uniform3(a.order + blockIdx.x) : (int)blockIdx.x;
     const int tx = b % a.ntx;
     b /= a.ntx;
     const int tyi = b % a.nty;
@@ -188,6 +192,62 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
         const int i = i0 + d;
         const bool live = i < zend;
         {
+        // ---- this plane's CPML memory variables, loaded before the LDS
+        // exchange so that their latency overlaps it (strip blocks and the
+        // bottom z strip only; uniform branches)
+        const int hz = i + N - 1;  // new A_z half point of this plane
+        const bool zh_in = nB > 0 && own && hz >= zb0 && hz < nz;
+        const bool zi_in = nB > 0 && own && live && i >= zb0;
+        float zph = 0.f, zbh = 0.f, zps = 0.f, zbi = 0.f;
+        if (zh_in) {
+            zph = a.zphi_in[((size_t)(hz - zb0) * ny + k) * nx + j];
+            zbh = a.zbm1[(size_t)(hz - zb0) * nx + j];
+        }
+        if (zi_in) {
+            zps = a.zpsi[((size_t)(i - zb0) * ny + k) * nx + j];
+            zbi = a.zbm1[(size_t)(i - zb0) * nx + j];
+        }
+        float xph[2] = {0.f, 0.f}, xbh[2] = {0.f, 0.f}, xps = 0.f, xbo = 0.f;
+        bool xh_in[2] = {false, false};
+        const bool xo_in = xstrip && own && live && (j < nB || j >= nx - nB);
+        if (xstrip) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int hx = x0 - N + lane + half * kTX;
+                xh_in[half] = live && k < ny && (half == 0 || lane < L) && hx >= 0 && hx < nx &&
+                              (hx < nB || hx >= nx - nB);
+                if (xh_in[half]) {
+                    const int xs = hx < nB ? hx : hx - (nx - nb2);
+                    xph[half] = a.xphi_in[((size_t)i * ny + k) * nb2 + xs];
+                    xbh[half] = a.xbm1[(size_t)i * nb2 + xs];
+                }
+            }
+            if (xo_in) {
+                const int xs = j < nB ? j : j - (nx - nb2);
+                xps = a.xpsi[((size_t)i * ny + k) * nb2 + xs];
+                xbo = a.xbm1[(size_t)i * nb2 + xs];
+            }
+        }
+        float yph[QY], ybh[QY], yps = 0.f, ybo = 0.f;
+        bool yh_in[QY];
+        const bool yo_in = ystrip && own && live && (k < nB || k >= ny - nB);
+#pragma unroll
+        for (int u = 0; u < QY; ++u) {
+            yph[u] = ybh[u] = 0.f;
+            const int hy = y0 - N + ty + u * TY;
+            yh_in[u] = ystrip && live && j < nx && ty + u * TY < AYH && hy >= 0 && hy < ny &&
+                       (hy < nB || hy >= ny - nB);
+            if (yh_in[u]) {
+                const int ys = hy < nB ? hy : hy - (ny - nb2);
+                yph[u] = a.yphi_in[((size_t)i * nb2 + ys) * nx + j];
+                ybh[u] = a.ybm1[(size_t)ys * nx + j];
+            }
+        }
+        if (yo_in) {
+            const int ys = k < nB ? k : k - (ny - nb2);
+            yps = a.ypsi[((size_t)i * nb2 + ys) * nx + j];
+            ybo = a.ybm1[(size_t)ys * nx + j];
+        }
         // ---- advance the z window to p(i-L .. i+L) and the A_z queue
 #pragma unroll
         for (int r = 0; r < 2 * L; ++r) zq[r] = zq[r + 1];
@@ -199,13 +259,11 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
             float dd = 0.f;
 #pragma unroll
             for (int m = 0; m < N; ++m) dd = fmaf(a.cz[m], zq[L + N + m] - zq[L + N - 1 - m], dd);
-            const int h = i + N - 1;
-            if (nB > 0 && own && h >= zb0 && h < nz) {
-                const size_t o = ((size_t)(h - zb0) * ny + k) * nx + j;
-                const float ph = a.zphi_in[o];
-                const float phi = fmaf(a.zbm1[(size_t)(h - zb0) * nx + j], ph + dd, ph);
+            if (zh_in) {
+                const float phi = fmaf(zbh, zph + dd, zph);
                 dd += phi;
-                if (h >= zbeg && h < zend) a.zphi_out[o] = phi;
+                if (hz >= zbeg && hz < zend)
+                    a.zphi_out[((size_t)(hz - zb0) * ny + k) * nx + j] = phi;
             }
             aq[2 * N - 1] = dd;
         }
@@ -218,11 +276,9 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
         float pz = 0.f;
 #pragma unroll
         for (int m = 0; m < N; ++m) pz = fmaf(a.cz[m], aq[N + m] - aq[N - 1 - m], pz);
-        if (nB > 0 && own && live && i >= zb0) {
-            const size_t o = ((size_t)(i - zb0) * ny + k) * nx + j;
-            const float ps = a.zpsi[o];
-            const float psi = fmaf(a.zbm1[(size_t)(i - zb0) * nx + j], ps + pz, ps);
-            a.zpsi[o] = psi;
+        if (zi_in) {
+            const float psi = fmaf(zbi, zps + pz, zps);
+            a.zpsi[((size_t)(i - zb0) * ny + k) * nx + j] = psi;
             pz += psi;
         }
 
@@ -244,67 +300,89 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
             for (int m = 1; m <= L; ++m) acc = fmaf(a.wy[m], c[m * PX] + c[-m * PX], acc);
             pxy = fmaf(a.w0lo, c[0], acc);  // low part of the centre weight, after cancellation
         } else {
-            // A_x at half points hx = x0-N+q (q < 64+L) for row ty
-            const float *row = P + (ty + L) * PX;
+            // damping strips: each direction takes the two-stage CPML form where
+            // its strip is within reach, its composite stencil otherwise
+            if (xstrip) {
+                // A_x at half points hx = x0-N+q (q < 64+L) for row ty
+                const float *row = P + (ty + L) * PX;
 #pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const int q = lane + half * kTX;
-                if (half == 1 && lane >= L) break;
-                float d = 0.f;
+                for (int half = 0; half < 2; ++half) {
+                    const int q = lane + half * kTX;
+                    if (half == 1 && lane >= L) break;
+                    float d = 0.f;
 #pragma unroll
-                for (int m = 0; m < N; ++m) d = fmaf(a.cx[m], row[q + N + m] - row[q + N - 1 - m], d);
-                const int hx = x0 - N + q;
-                if (xstrip && live && k < ny && hx >= 0 && hx < nx && (hx < nB || hx >= nx - nB)) {
-                    const int xs = hx < nB ? hx : hx - (nx - nb2);
-                    const size_t o = ((size_t)i * ny + k) * nb2 + xs;
-                    const float ph = a.xphi_in[o];
-                    const float phi = fmaf(a.xbm1[(size_t)i * nb2 + xs], ph + d, ph);
-                    d += phi;
-                    if (hx >= x0 && hx < x0 + kTX) a.xphi_out[o] = phi;
+                    for (int m = 0; m < N; ++m) d = fmaf(a.cx[m], row[q + N + m] - row[q + N - 1 - m], d);
+                    const int hx = x0 - N + q;
+                    if (xh_in[half]) {
+                        const float phi = fmaf(xbh[half], xph[half] + d, xph[half]);
+                        d += phi;
+                        if (hx >= x0 && hx < x0 + kTX) {
+                            const int xs = hx < nB ? hx : hx - (nx - nb2);
+                            a.xphi_out[((size_t)i * ny + k) * nb2 + xs] = phi;
+                        }
+                    }
+                    sax[ty * AXW + q] = d;
                 }
-                sax[ty * AXW + q] = d;
             }
-            // A_y at half points hy = y0-N+q (q < TY+L) for column lane
-            for (int q = ty; q < AYH; q += TY) {
-                const float *cl = P + lane + L;
-                float d = 0.f;
+            if (ystrip) {
+                // A_y at half points hy = y0-N+q (q < TY+L) for column lane
 #pragma unroll
-                for (int m = 0; m < N; ++m)
-                    d = fmaf(a.cy[m], cl[(q + N + m) * PX] - cl[(q + N - 1 - m) * PX], d);
-                const int hy = y0 - N + q;
-                if (ystrip && live && j < nx && hy >= 0 && hy < ny && (hy < nB || hy >= ny - nB)) {
-                    const int ys = hy < nB ? hy : hy - (ny - nb2);
-                    const size_t o = ((size_t)i * nb2 + ys) * nx + j;
-                    const float ph = a.yphi_in[o];
-                    const float phi = fmaf(a.ybm1[(size_t)ys * nx + j], ph + d, ph);
-                    d += phi;
-                    if (hy >= y0 && hy < y0 + TY) a.yphi_out[o] = phi;
+                for (int u = 0; u < QY; ++u) {
+                    const int q = ty + u * TY;
+                    if (q >= AYH) break;
+                    const float *cl = P + lane + L;
+                    float d = 0.f;
+#pragma unroll
+                    for (int m = 0; m < N; ++m)
+                        d = fmaf(a.cy[m], cl[(q + N + m) * PX] - cl[(q + N - 1 - m) * PX], d);
+                    const int hy = y0 - N + q;
+                    if (yh_in[u]) {
+                        const float phi = fmaf(ybh[u], yph[u] + d, yph[u]);
+                        d += phi;
+                        if (hy >= y0 && hy < y0 + TY) {
+                            const int ys = hy < nB ? hy : hy - (ny - nb2);
+                            a.yphi_out[((size_t)i * nb2 + ys) * nx + j] = phi;
+                        }
+                    }
+                    say[q * kTX + lane] = d;
                 }
-                say[q * kTX + lane] = d;
             }
-            __syncthreads();
-            float px = 0.f, py = 0.f;
-            const float *ax = sax + ty * AXW + lane + N;
+            __syncthreads();  // uniform: every strip block exchanges A in LDS
+            const float *c = P + (ty + L) * PX + lane + L;
+            float px, py;
+            if (xstrip) {
+                px = 0.f;
+                const float *ax = sax + ty * AXW + lane + N;
 #pragma unroll
-            for (int m = 0; m < N; ++m) px = fmaf(a.cx[m], ax[m] - ax[-1 - m], px);
-            const float *ay = say + (ty + N) * kTX + lane;
+                for (int m = 0; m < N; ++m) px = fmaf(a.cx[m], ax[m] - ax[-1 - m], px);
+                if (xo_in) {
+                    const int xs = j < nB ? j : j - (nx - nb2);
+                    const float psi = fmaf(xbo, xps + px, xps);
+                    a.xpsi[((size_t)i * ny + k) * nb2 + xs] = psi;
+                    px += psi;
+                }
+            } else {
+                float acc = a.w0x * c[0];
 #pragma unroll
-            for (int m = 0; m < N; ++m) py = fmaf(a.cy[m], ay[m * kTX] - ay[(-1 - m) * kTX], py);
-            if (xstrip && own && live && (j < nB || j >= nx - nB)) {
-                const int xs = j < nB ? j : j - (nx - nb2);
-                const size_t o = ((size_t)i * ny + k) * nb2 + xs;
-                const float ps = a.xpsi[o];
-                const float psi = fmaf(a.xbm1[(size_t)i * nb2 + xs], ps + px, ps);
-                a.xpsi[o] = psi;
-                px += psi;
+                for (int m = 1; m <= L; ++m) acc = fmaf(a.wx[m], c[m] + c[-m], acc);
+                px = fmaf(a.w0xlo, c[0], acc);
             }
-            if (ystrip && own && live && (k < nB || k >= ny - nB)) {
-                const int ys = k < nB ? k : k - (ny - nb2);
-                const size_t o = ((size_t)i * nb2 + ys) * nx + j;
-                const float ps = a.ypsi[o];
-                const float psi = fmaf(a.ybm1[(size_t)ys * nx + j], ps + py, ps);
-                a.ypsi[o] = psi;
-                py += psi;
+            if (ystrip) {
+                py = 0.f;
+                const float *ay = say + (ty + N) * kTX + lane;
+#pragma unroll
+                for (int m = 0; m < N; ++m) py = fmaf(a.cy[m], ay[m * kTX] - ay[(-1 - m) * kTX], py);
+                if (yo_in) {
+                    const int ys = k < nB ? k : k - (ny - nb2);
+                    const float psi = fmaf(ybo, yps + py, yps);
+                    a.ypsi[((size_t)i * nb2 + ys) * nx + j] = psi;
+                    py += psi;
+                }
+            } else {
+                float acc = a.w0y * c[0];
+#pragma unroll
+                for (int m = 1; m <= L; ++m) acc = fmaf(a.wy[m], c[m * PX] + c[-m * PX], acc);
+                py = fmaf(a.w0ylo, c[0], acc);
             }
             pxy = px + py;
         }
@@ -362,7 +440,10 @@ class Prop3D final : public Prop {
     DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1, yphi[2], ypsi, ybm1;
     int cur = 0, phase = 0;
     float cz[8] = {0}, cx[8] = {0}, cy[8] = {0}, w0 = 0.f, w0lo = 0.f, wx[16] = {0}, wy[16] = {0};
+    float w0x = 0.f, w0xlo = 0.f, w0y = 0.f, w0ylo = 0.f;
     int ty = 8, zc = 64;
+    DevBuf<int> order;  // block launch order for (order_ty, order_zc)
+    int order_ty = 0, order_zc = 0;
 
     Prop3D(const double *v, int64_t nz_, int64_t nx_, int64_t ny_, int N_, int nB_, double dz_,
            double dx_, double dy_, double dt_) {
@@ -411,15 +492,22 @@ class Prop3D final : public Prop {
         double w[32];
         composite_weights3(N, c, w);
         const int L = 2 * N - 1;
-        double side = 0.0;  // exact in double
+        double side = 0.0, sidex = 0.0, sidey = 0.0;  // exact in double
         for (int m = 1; m <= L; ++m) {
             wx[m] = (float)(w[L + m] / (dx * dx));
             wy[m] = (float)(w[L + m] / (dy * dy));
             side += 2.0 * ((double)wx[m] + (double)wy[m]);
+            sidex += 2.0 * (double)wx[m];
+            sidey += 2.0 * (double)wy[m];
         }
         // centre weight makes the FP32 weights sum to zero, carried as w0 + w0lo
+        // (per direction for strip blocks that use one composite direction)
         w0 = (float)(-side);
         w0lo = (float)(-side - (double)w0);
+        w0x = (float)(-sidex);
+        w0xlo = (float)(-sidex - (double)w0x);
+        w0y = (float)(-sidey);
+        w0ylo = (float)(-sidey - (double)w0y);
         if (nB > 0) {
             // fwdTimeCpmlFor3dAw.m:46-70 with dampPml.m: u in samples, L = nB
             const double logR = std::log(1e-6), Lb = (double)nB;
@@ -571,6 +659,40 @@ class Prop3D final : public Prop {
         hipLaunchKernelGGL((cpml3d_step<NN, TY>), dim3(nblk), dim3(64 * TY), 0, stream, args);
     }
 
+    // Blocks whose x or y stencil reach touches a damping strip go first (they
+    // run the two-stage form and carry the strip arrays' traffic); the others
+    // follow in 8 contiguous bands, block b landing on XCD b % 8.
+    void build_order(int ntx, int nty, int nzc) {
+        const int nblk = ntx * nty * nzc;
+        std::vector<int> strip, inner;
+        for (int b = 0; b < nblk; ++b) {
+            const int tx = b % ntx, tyi = (b / ntx) % nty;
+            const int64_t x0 = (int64_t)tx * kTX, y0 = (int64_t)tyi * ty;
+            const bool xs = nB > 0 && (x0 - N < nB || x0 + kTX + N - 1 > nx - nB);
+            const bool ys = nB > 0 && (y0 - N < nB || y0 + ty + N - 1 > ny - nB);
+            (xs || ys ? strip : inner).push_back(b);
+        }
+        std::vector<int> ord(strip);
+        const size_t n = inner.size(), base = ord.size();
+        size_t next[8], end[8];
+        for (int x = 0; x < 8; ++x) {
+            next[x] = n * x / 8;
+            end[x] = n * (x + 1) / 8;
+        }
+        for (size_t b = base; b < base + n; ++b) {
+            int x = (int)(b % 8);
+            if (next[x] == end[x])
+                for (int y = 0; y < 8; ++y)
+                    if (next[y] < end[y]) { x = y; break; }
+            ord.push_back(inner[next[x]++]);
+        }
+        SSSI_HIP(hipStreamSynchronize(stream));  // the previous order may be in use
+        order.alloc(ord.size());
+        SSSI_HIP(hipMemcpy(order.p, ord.data(), order.bytes(), hipMemcpyHostToDevice));
+        order_ty = ty;
+        order_zc = zc;
+    }
+
     void step(int64_t t, bool record, float *snap) {
         Cpml3dArgs a{};
         a.pcur = p[cur].p;
@@ -587,6 +709,10 @@ class Prop3D final : public Prop {
         for (int q = 0; q < 8; ++q) { a.cz[q] = cz[q]; a.cx[q] = cx[q]; a.cy[q] = cy[q]; }
         a.w0 = w0;
         a.w0lo = w0lo;
+        a.w0x = w0x;
+        a.w0xlo = w0xlo;
+        a.w0y = w0y;
+        a.w0ylo = w0ylo;
         for (int m = 0; m < 16; ++m) { a.wx[m] = wx[m]; a.wy[m] = wy[m]; }
         if (nB > 0) {
             a.zphi_in = zphi[phase].p; a.zphi_out = zphi[phase ^ 1].p; a.zpsi = zpsi.p; a.zbm1 = zbm1.p;
@@ -602,6 +728,8 @@ class Prop3D final : public Prop {
         a.snap = snap;
         const int nzc = (int)((nz + zc - 1) / zc);
         const int nblk = a.ntx * a.nty * nzc;
+        if (order_ty != ty || order_zc != zc) build_order(a.ntx, a.nty, nzc);
+        a.order = order.p;
         switch (N * 100 + ty) {
 #define SSSI_C3(NN, T) case NN * 100 + T: launch<NN, T>(a, nblk); break;
             SSSI_C3(1, 4) SSSI_C3(1, 8) SSSI_C3(1, 16) SSSI_C3(2, 4) SSSI_C3(2, 8) SSSI_C3(2, 16)
