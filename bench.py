@@ -45,10 +45,10 @@ def workload():
                 src_ix=[nx0 // 2 + nB])
 
 
-def traffic_from_profiles():
+def traffic_from_profiles(name="cpml2d_step_bench_traffic.json"):
     """HBM bytes per launch of the time-step kernel, from the committed rocprof
     counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
-    path = os.path.join(ROOT, "profiles", "cpml2d_step_bench_traffic.json")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             return json.load(f).get("hbm_bytes_per_launch")
@@ -123,7 +123,8 @@ def bench_3d(args):
                    "parallelism": "1 GPU",
                    "layout": {"block_y": ty, "z_chunk": zc, "autotuned": not args.no_tune}},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("cpml3d_step_bench_traffic.json"),
                      "kernel": "cpml3d_step", "alg_bytes_per_launch": bytes_step,
                      "avg_launch_us": round(avg * 1e6, 3), "launches": launches},
     }

@@ -72,6 +72,19 @@ __device__ __forceinline__ T ld_uniform3(const T *p) {
 __device__ __forceinline__ float ld3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
+// Streams touched once per step (p_old, v^2dt^2 reads; the p_new store) carry
+// the non-temporal cache policy so that they do not evict the p planes that
+// neighbouring blocks re-read as their x/y halo from L2.
+constexpr int kNonTemporal = 2;  // cache-policy NT bit of gfx950 buffer operations
+__device__ __forceinline__ float ld3_nt(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(
+        __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, kNonTemporal));
+}
+__device__ __forceinline__ void st3_nt(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                       uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff,
+                                          kNonTemporal);
+}
 __device__ __forceinline__ void st3(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
                                     uint32_t soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
@@ -176,8 +189,8 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
     for (int d = 0; d < PD; ++d) {
         const int ip = min(zbeg + d, nz - 1);  // in bounds; unused past zend
         f_front[d] = ld3(rP, vcol, plane(ip + L));
-        f_po[d] = ld3(rO, vcol, plane(ip));
-        f_v2[d] = ld3(rV, vcol, plane(ip));
+        f_po[d] = ld3_nt(rO, vcol, plane(ip));
+        f_v2[d] = ld3_nt(rV, vcol, plane(ip));
 #pragma unroll
         for (int u = 0; u < HPT; ++u) f_h[d][u] = ld3(rP, h_voff[u], plane(ip));
     }
@@ -400,7 +413,7 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
             }
             const float pc = zq[L];
             const float pn = pc + fmaf(v2, (pz + pxy) + s, pc - po);
-            st3(pn, rO, vcol, plane(i));
+            st3_nt(pn, rO, vcol, plane(i));
             if (a.rec && i == 0) a.rec[(size_t)k * nx + j] = pn;
             if (a.snap) a.snap[((size_t)i * ny + k) * nx + j] = pn;
         }
@@ -409,8 +422,8 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
         // the loop back-edge); past the chunk they read the bottom padding
         // (2*PD + L <= kHZbot)
         f_front[d] = ld3(rP, vcol, plane(i + PD + L));
-        f_po[d] = ld3(rO, vcol, plane(i + PD));
-        f_v2[d] = ld3(rV, vcol, plane(i + PD));
+        f_po[d] = ld3_nt(rO, vcol, plane(i + PD));
+        f_v2[d] = ld3_nt(rV, vcol, plane(i + PD));
 #pragma unroll
         for (int u = 0; u < HPT; ++u) f_h[d][u] = ld3(rP, h_voff[u], plane(i + PD));
         buf ^= 1;
