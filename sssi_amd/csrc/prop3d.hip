@@ -360,7 +360,15 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
                     say[q * kTX + lane] = d;
                 }
             }
-            __syncthreads();  // uniform: every strip block exchanges A in LDS
+            // A_x rows are written and read by the same wavefront (row ty); A_y
+            // columns cross wavefronts and need the block barrier
+            if (ystrip) {
+                __syncthreads();
+            } else {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
             const float *c = P + (ty + L) * PX + lane + L;
             float px, py;
             if (xstrip) {
