@@ -181,6 +181,18 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
         aq[q + 1] = d;
     }
 
+    // y-strip b-1 values do not change along z: loaded once per block
+    float ybh[QY], ybo = 0.f;
+#pragma unroll
+    for (int u = 0; u < QY; ++u) {
+        ybh[u] = 0.f;
+        const int hy = y0 - N + ty + u * TY;
+        if (ystrip && j < nx && ty + u * TY < AYH && hy >= 0 && hy < ny && (hy < nB || hy >= ny - nB))
+            ybh[u] = a.ybm1[(size_t)(hy < nB ? hy : hy - (ny - nb2)) * nx + j];
+    }
+    if (ystrip && own && (k < nB || k >= ny - nB))
+        ybo = a.ybm1[(size_t)(k < nB ? k : k - (ny - nb2)) * nx + j];
+
     // prefetch ring: planes zbeg .. zbeg+PD-1 (slot d holds plane i with
     // (i - zbeg) % PD == d); a slot is refilled right after it is consumed, so
     // each plane's loads have PD plane-iterations to arrive
@@ -241,25 +253,23 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
                 xbo = a.xbm1[(size_t)i * nb2 + xs];
             }
         }
-        float yph[QY], ybh[QY], yps = 0.f, ybo = 0.f;
+        float yph[QY], yps = 0.f;
         bool yh_in[QY];
         const bool yo_in = ystrip && own && live && (k < nB || k >= ny - nB);
 #pragma unroll
         for (int u = 0; u < QY; ++u) {
-            yph[u] = ybh[u] = 0.f;
+            yph[u] = 0.f;
             const int hy = y0 - N + ty + u * TY;
             yh_in[u] = ystrip && live && j < nx && ty + u * TY < AYH && hy >= 0 && hy < ny &&
                        (hy < nB || hy >= ny - nB);
             if (yh_in[u]) {
                 const int ys = hy < nB ? hy : hy - (ny - nb2);
                 yph[u] = a.yphi_in[((size_t)i * nb2 + ys) * nx + j];
-                ybh[u] = a.ybm1[(size_t)ys * nx + j];
             }
         }
         if (yo_in) {
             const int ys = k < nB ? k : k - (ny - nb2);
             yps = a.ypsi[((size_t)i * nb2 + ys) * nx + j];
-            ybo = a.ybm1[(size_t)ys * nx + j];
         }
         // ---- advance the z window to p(i-L .. i+L) and the A_z queue
 #pragma unroll
