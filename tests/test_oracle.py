@@ -17,7 +17,9 @@ Checks, per oracle function:
   sponge fwd/rvs      bit-exact vs the restatement; closed-form damping factor
   imaging             brute-force cross-correlation sums
 """
+import json
 import math
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -27,12 +29,17 @@ import oracle
 import matlab_restatement as M
 from sssi_amd import synth
 
-# UserGuide.tex:339-341 -- the reference's own table of staggered coefficients
-USERGUIDE_COEFFS = {
-    1: [Fraction(1)],
-    2: [Fraction(9, 8), Fraction(-1, 24)],
-    3: [Fraction(75, 64), Fraction(-25, 384), Fraction(3, 640)],
-}
+# UserGuide.tex:339-341 -- the reference's own table of staggered coefficients,
+# committed as a fixture
+def _userguide_coeffs():
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "staggered_coefficients.json")
+    with open(path) as f:
+        table = json.load(f)["coefficients"]
+    return {int(N): [Fraction(a, b) for a, b in row] for N, row in table.items()}
+
+
+USERGUIDE_COEFFS = _userguide_coeffs()
 
 
 def test_dcoef_userguide_table():
