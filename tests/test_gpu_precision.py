@@ -62,3 +62,27 @@ def test_3d_precision():
           + ", ".join(f"{e:.3e}" for e in e_snap), flush=True)
     assert e_data < 1e-5
     assert max(e_snap) < 5e-5
+
+
+@pytest.mark.parametrize("fmt,tol", [("fp32", 1e-5), ("bf16", 1e-3)])
+def test_rtm_image_precision_at_benchmark_size(fmt, tol):
+    """One shot of the 64-shot RTM workload (600 x 6000 model, window of the
+    first shot, nDiffOrder 4, 3000 steps, imaging every 5 steps) against the
+    FP64 oracle."""
+    import bench
+    wl = bench.workload_rtm(1)
+    xb, xc, s = wl["shots"][0]
+    r = sssi_amd.Rtm2D(wl["V"], wl["VS"], wl["N"], wl["nB"], wl["d"], wl["d"], wl["dt"],
+                       wl["nt"], img_stride=5, ck_format=fmt)
+    r.add_shot(xb, xc, s, wl["wavelet"])
+    r.migrate(batched=True)
+    g = r.shot(0)
+    r.close()
+    Vw = synth.extBoundary(wl["V"][:, xb:xb + xc], wl["nB"], 2)
+    VSw = synth.extBoundary(wl["VS"][:, xb:xb + xc], wl["nB"], 2)
+    o = oracle.rtm_shot_cpml_2d(Vw, VSw, wl["nt"], [0], [s + wl["nB"]], wl["wavelet"], wl["N"],
+                                wl["nB"], wl["d"], wl["d"], wl["dt"], img_stride=5, eps=1e-3)
+    e = rel_l2(g["image"], o["image"])
+    print(f"RTM {Vw.shape[0]}x{Vw.shape[1]} nt={wl['nt']} {fmt} checkpoints: image rel-L2 {e:.3e}",
+          flush=True)
+    assert e < tol
