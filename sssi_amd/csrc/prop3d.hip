@@ -10,12 +10,16 @@
 // column tile and sweeps a chunk of ZC planes downwards (2.5-D): the z
 // derivative runs from registers (a window of 2L+1 p values and a queue of
 // 2N A_z half-point values per column, so each D+ is computed once), x and y
-// from a double-buffered LDS plane tile with an L-wide halo.  Column tiles
-// whose x/y reach touches no damping strip use the composite D-D+ stencil in
-// x and y; strip tiles run the two-stage CPML form (A_x, A_y at half points
-// through LDS).  The next plane's global loads are issued before the current
-// plane is computed.  Phi is ping-ponged (neighbouring tiles/chunks recompute
-// the half points at their edges), Psi is updated in place.
+// from a double-buffered LDS plane tile with an L-wide halo.  The planes'
+// global loads run PD planes ahead in a register ring.
+//
+// The z chunks are split at the depth where the z stencil first reaches the
+// bottom damping strip, so that a block either touches no strip at all (the
+// interior sweep: composite D-D+ stencil in x, y and z, nothing but the ring
+// in flight) or is a strip block (x/y strip tile or bottom chunk: the two-stage
+// CPML form in the directions whose strip is within reach, A_x and A_y at half
+// points through LDS; Phi ping-ponged, neighbouring tiles and chunks
+// recomputing the half points at their edges; Psi in place).
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -35,22 +39,23 @@ struct Cpml3dArgs {
     const float *vdt2;
     int nz, nx, ny, pitch, npy, nB;
     int ntx, nty, zc;  // tile counts in x, y; planes per chunk
+    // z chunks: nzA chunks tile [0, zsplit), whose z reach stays above the
+    // bottom damping strip; the rest tile [zsplit, nz)
+    int zsplit, nzA;
     uint32_t rowb, planeb, nbytes;
     float cz[8], cx[8], cy[8];
     float w0, w0lo, wx[16], wy[16];  // composite x/y weights / d^2 (w0 + w0lo: centre)
     float w0x, w0xlo, w0y, w0ylo;    // centres of the x / y stencils alone
-    // z strip: half points / rows i in [nz-nB, nz): [(i-(nz-nB))*ny + k]*nx + j
-    const float *zphi_in;
-    float *zphi_out, *zpsi;
-    const float *zbm1;  // [(i-(nz-nB))*nx + j]  (replicated over y)
-    // x strip: [(i*ny + k)*2nB + xs]
-    const float *xphi_in;
-    float *xphi_out, *xpsi;
-    const float *xbm1;  // [i*2nB + xs]           (replicated over y)
-    // y strip: [(i*2nB + ys)*nx + j]
-    const float *yphi_in;
-    float *yphi_out, *ypsi;
-    const float *ybm1;  // [ys*nx + j]            (replicated over z)
+    // Damping-strip arrays, byte offsets into one pool of pool_bytes < 2 GiB:
+    // z strip: half points / rows i in [nz-nB, nz): [(i-(nz-nB))*ny + k]*nx + j,
+    //   b-1 at [(i-(nz-nB))*nx + j] (replicated over y);
+    // x strip: [(i*ny + k)*2nB + xs], b-1 at [i*2nB + xs] (replicated over y);
+    // y strip: [(i*2nB + ys)*nx + j], b-1 at [ys*nx + j] (replicated over z).
+    float *pool;
+    uint32_t pool_bytes;
+    uint32_t o_zphi_in, o_zphi_out, o_zpsi, o_zbm1;
+    uint32_t o_xphi_in, o_xphi_out, o_xpsi, o_xbm1;
+    uint32_t o_yphi_in, o_yphi_out, o_ypsi, o_ybm1;
     const int *order;     // launch order of the blocks (strip blocks first) or null
     const int *src_cell;  // (i*ny + k)*nx + j, sorted
     const float *src_amp;
@@ -93,51 +98,49 @@ __device__ __forceinline__ void st3(float v, __amdgpu_buffer_rsrc_t r, uint32_t 
 constexpr int kPD3 = 2;  // planes prefetched ahead by the 3-D sweep
 static_assert(2 * kPD3 + 15 <= kHZbot, "bottom padding must hold the sweep's overrun");
 
-template <int N, int TY, int PD = kPD3>
-__global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
-    constexpr int L = 2 * N - 1;
-    constexpr int NT = 64 * TY;
-    constexpr int PX = kTX + 2 * L;  // LDS plane tile: x0-L .. x0+63+L
-    constexpr int PY = TY + 2 * L;   //                 y0-L .. y0+TY-1+L
-    constexpr int HN = PX * PY - kTX * TY;  // halo values per plane
-    constexpr int HPT = (HN + NT - 1) / NT;
-    constexpr int AXW = kTX + L;  // A_x half points x0-N .. x0+63+N-1 per row
-    constexpr int AYH = TY + L;   // A_y half points y0-N .. y0+TY-1+N-1 per column
-    constexpr int QY = (AYH + TY - 1) / TY;  // A_y half points per thread
-    __shared__ float sp[2][PX * PY];
-    __shared__ float sax[TY * AXW];
-    __shared__ float say[AYH * kTX];
+// Strip state lives in one pool behind one buffer resource; a lane without a
+// value in a strip array takes this byte offset, past the end of any pool
+// (pools stay below 2 GiB): its buffer load returns 0 and its store is dropped,
+// so the strip code runs without per-lane branches.
+constexpr uint32_t kOOB = 0x80000000u;
+__device__ __forceinline__ float ldS(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0));
+}
+__device__ __forceinline__ void stS(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, 0, 0);
+}
 
-    const int tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
-    int b = a.order ? ld_uniform3(a.order + blockIdx.x) : (int)blockIdx.x;
-    const int tx = b % a.ntx;
-    b /= a.ntx;
-    const int tyi = b % a.nty;
-    const int tz = b / a.nty;
-    const int x0 = tx * kTX, y0 = tyi * TY;
-    const int nz = a.nz, nx = a.nx, ny = a.ny, nB = a.nB;
-    const int zbeg = tz * a.zc, zend = min(zbeg + a.zc, nz);
-    const int j = x0 + lane, k = y0 + ty;
-    const bool own = j < nx && k < ny;
-    const __amdgpu_buffer_rsrc_t rP = buf_rsrc3(a.pcur, a.nbytes);
-    const __amdgpu_buffer_rsrc_t rO = buf_rsrc3(a.pold, a.nbytes);
-    const __amdgpu_buffer_rsrc_t rV = buf_rsrc3(a.vdt2, a.nbytes);
-    const uint32_t vcol = ((uint32_t)(k + kHY) * a.pitch + (uint32_t)(j + kHX)) * 4u;
-    auto plane = [&](int i) -> uint32_t { return (uint32_t)(i + kHZ) * a.planeb; };
+// Tile geometry shared by both sweeps.
+template <int N, int TY>
+struct Tile3 {
+    static constexpr int L = 2 * N - 1;
+    static constexpr int NT = 64 * TY;
+    static constexpr int PX = kTX + 2 * L;  // LDS plane tile: x0-L .. x0+63+L
+    static constexpr int PY = TY + 2 * L;   //                 y0-L .. y0+TY-1+L
+    static constexpr int HN = PX * PY - kTX * TY;  // halo values per plane
+    static constexpr int HPT = (HN + NT - 1) / NT;
+    static constexpr int AXW = kTX + L;  // A_x half points x0-N .. x0+63+N-1 per row
+    static constexpr int AYH = TY + L;   // A_y half points y0-N .. y0+TY-1+N-1 per column
+    static constexpr int QY = (AYH + TY - 1) / TY;  // A_y half points per thread
+};
 
-    // halo slots of this thread in the LDS plane tile (fixed for the sweep);
-    // threads without a slot load a valid cell that is never written to LDS
-    // (an unconditional load keeps the prefetched registers out of branches)
-    int h_lds[HPT];
-    uint32_t h_voff[HPT];
+// Halo slots of this thread in the LDS plane tile (fixed for the sweep).
+// Threads without a slot load a valid cell that is never written to LDS (an
+// unconditional load keeps the prefetched registers out of branches).
+template <int N, int TY>
+__device__ __forceinline__ void halo_slots3(const Cpml3dArgs &a, int tid, int x0, int y0,
+                                            int (&h_lds)[Tile3<N, TY>::HPT],
+                                            uint32_t (&h_voff)[Tile3<N, TY>::HPT]) {
+    using T = Tile3<N, TY>;
+    constexpr int L = T::L, PX = T::PX;
 #pragma unroll
-    for (int u = 0; u < HPT; ++u) {
-        const int e = tid + NT * u;
+    for (int u = 0; u < T::HPT; ++u) {
+        const int e = tid + T::NT * u;
         int lidx = -1, gy = 0, gx = 0;
-        if (e < HN) {
+        if (e < T::HN) {
             // enumerate tile cells that are not owned: rows of the y halo first
-            const int top = L * PX;                 // y0-L .. y0-1
-            const int mid = TY * 2 * L;             // x halo of the owned rows
+            const int top = L * PX;      // y0-L .. y0-1
+            const int mid = TY * 2 * L;  // x halo of the owned rows
             if (e < top) {
                 const int r = e / PX, c = e - r * PX;
                 lidx = r * PX + c; gy = y0 - L + r; gx = x0 - L + c;
@@ -153,178 +156,333 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
         h_lds[u] = lidx;
         h_voff[u] = ((uint32_t)(gy + kHY) * a.pitch + (uint32_t)(gx + kHX)) * 4u;
     }
+}
 
-    const bool xstrip = nB > 0 && (x0 - N < nB || x0 + kTX + N - 1 > nx - nB);
-    const bool ystrip = nB > 0 && (y0 - N < nB || y0 + TY + N - 1 > ny - nB);
-    const int zb0 = nz - nB;
-    const int nb2 = 2 * nB;
+// Sweep of a column tile and z chunk whose stencil reach touches no damping
+// strip: the composite D-D+ stencil in x, y and z, no CPML state, and the only
+// global loads in the plane loop are the prefetch ring's, so the in-order
+// vmcnt waits never drain the ring.
+template <int N, int TY, int PD>
+__device__ __forceinline__ void cpml3d_sweep_inner(const Cpml3dArgs &a, float *sp0, int tx, int tyi,
+                                                   int zbeg, int zend) {
+    using T = Tile3<N, TY>;
+    constexpr int L = T::L, PX = T::PX, PY = T::PY, HPT = T::HPT;
+    const int tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
+    const int x0 = tx * kTX, y0 = tyi * TY;
+    const int nz = a.nz, nx = a.nx, ny = a.ny;
+    const int j = x0 + lane, k = y0 + ty;
+    const bool own = j < nx && k < ny;
+    const __amdgpu_buffer_rsrc_t rP = buf_rsrc3(a.pcur, a.nbytes);
+    const __amdgpu_buffer_rsrc_t rO = buf_rsrc3(a.pold, a.nbytes);
+    const __amdgpu_buffer_rsrc_t rV = buf_rsrc3(a.vdt2, a.nbytes);
+    const uint32_t vcol = ((uint32_t)(k + kHY) * a.pitch + (uint32_t)(j + kHX)) * 4u;
+    auto plane = [&](int i) -> uint32_t { return (uint32_t)(i + kHZ) * a.planeb; };
+    int h_lds[HPT];
+    uint32_t h_voff[HPT];
+    halo_slots3<N, TY>(a, tid, x0, y0, h_lds, h_voff);
 
     // z window p(i-L .. i+L) and A_z queue (half points i-N .. i+N-1)
     float zq[2 * L + 1], aq[2 * N];
 #pragma unroll
     for (int r = 0; r < 2 * L; ++r) zq[r + 1] = ld3(rP, vcol, plane(zbeg - L + r));
-    // A_z at half points zbeg-N .. zbeg+N-2 (aq[1..2N-1])
 #pragma unroll
     for (int q = 0; q < 2 * N - 1; ++q) {
         // half point h = zbeg-N+q uses p(h-N+1 .. h+N) = zq[1+q .. 1+q+2N-1]
         float d = 0.f;
 #pragma unroll
         for (int m = 0; m < N; ++m) d = fmaf(a.cz[m], zq[1 + q + N + m] - zq[q + N - m], d);
-        const int h = zbeg - N + q;
-        if (nB > 0 && own && h >= zb0 && h < nz) {
-            const size_t o = ((size_t)(h - zb0) * ny + k) * nx + j;
-            const float phi = fmaf(a.zbm1[(size_t)(h - zb0) * nx + j], a.zphi_in[o] + d,
-                                   a.zphi_in[o]);
-            d += phi;
-            if (h >= zbeg && h < zend) a.zphi_out[o] = phi;
-        }
         aq[q + 1] = d;
     }
 
-    // y-strip b-1 values do not change along z: loaded once per block
-    float ybh[QY], ybo = 0.f;
-#pragma unroll
-    for (int u = 0; u < QY; ++u) {
-        ybh[u] = 0.f;
-        const int hy = y0 - N + ty + u * TY;
-        if (ystrip && j < nx && ty + u * TY < AYH && hy >= 0 && hy < ny && (hy < nB || hy >= ny - nB))
-            ybh[u] = a.ybm1[(size_t)(hy < nB ? hy : hy - (ny - nb2)) * nx + j];
-    }
-    if (ystrip && own && (k < nB || k >= ny - nB))
-        ybo = a.ybm1[(size_t)(k < nB ? k : k - (ny - nb2)) * nx + j];
-
     // prefetch ring: planes zbeg .. zbeg+PD-1 (slot d holds plane i with
     // (i - zbeg) % PD == d); a slot is refilled right after it is consumed, so
-    // each plane's loads have PD plane-iterations to arrive
+    // each plane's loads have PD plane-iterations to arrive.  The slots are
+    // issued in slot order (scheduling barriers): vmcnt counts in issue order,
+    // and the loop's waits must also hold on entry from here, so a slot-0 load
+    // scheduled after slot 1's would make every iteration wait for all loads.
     float f_front[PD], f_po[PD], f_v2[PD], f_h[PD][HPT];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int d = 0; d < PD; ++d) {
         const int ip = min(zbeg + d, nz - 1);  // in bounds; unused past zend
-        f_front[d] = ld3(rP, vcol, plane(ip + L));
         f_po[d] = ld3_nt(rO, vcol, plane(ip));
         f_v2[d] = ld3_nt(rV, vcol, plane(ip));
+        f_front[d] = ld3(rP, vcol, plane(ip + L));
 #pragma unroll
         for (int u = 0; u < HPT; ++u) f_h[d][u] = ld3(rP, h_voff[u], plane(ip));
+        __builtin_amdgcn_sched_barrier(0);
     }
 
     int buf = 0;
     for (int i0 = zbeg; i0 < zend; i0 += PD) {
 #pragma unroll
-    for (int d = 0; d < PD; ++d) {
-        // planes past the chunk end (last group of a chunk) run on the zero
-        // padding below the model: nothing is stored and no strip array is
-        // touched for them, so the sweep stays free of per-plane branches
-        const int i = i0 + d;
-        const bool live = i < zend;
-        {
-        // ---- this plane's CPML memory variables, loaded before the LDS
-        // exchange so that their latency overlaps it (strip blocks and the
-        // bottom z strip only; uniform branches)
-        const int hz = i + N - 1;  // new A_z half point of this plane
-        const bool zh_in = nB > 0 && own && hz >= zb0 && hz < nz;
-        const bool zi_in = nB > 0 && own && live && i >= zb0;
-        float zph = 0.f, zbh = 0.f, zps = 0.f, zbi = 0.f;
-        if (zh_in) {
-            zph = a.zphi_in[((size_t)(hz - zb0) * ny + k) * nx + j];
-            zbh = a.zbm1[(size_t)(hz - zb0) * nx + j];
-        }
-        if (zi_in) {
-            zps = a.zpsi[((size_t)(i - zb0) * ny + k) * nx + j];
-            zbi = a.zbm1[(size_t)(i - zb0) * nx + j];
-        }
-        float xph[2] = {0.f, 0.f}, xbh[2] = {0.f, 0.f}, xps = 0.f, xbo = 0.f;
-        bool xh_in[2] = {false, false};
-        const bool xo_in = xstrip && own && live && (j < nB || j >= nx - nB);
-        if (xstrip) {
+        for (int d = 0; d < PD; ++d) {
+            // planes past the chunk end (last group of a chunk) run on the zero
+            // padding below the model and store nothing
+            const int i = i0 + d;
+            const bool live = i < zend;
+            // advance the z window; the ring's front value enters it through a
+            // move, so that the window owns a register of its own and the
+            // refill below loads into the slot's register (no copy of an
+            // in-flight load at the loop back-edge)
 #pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const int hx = x0 - N + lane + half * kTX;
-                xh_in[half] = live && k < ny && (half == 0 || lane < L) && hx >= 0 && hx < nx &&
-                              (hx < nB || hx >= nx - nB);
-                if (xh_in[half]) {
-                    const int xs = hx < nB ? hx : hx - (nx - nb2);
-                    xph[half] = a.xphi_in[((size_t)i * ny + k) * nb2 + xs];
-                    xbh[half] = a.xbm1[(size_t)i * nb2 + xs];
-                }
+            for (int r = 0; r < 2 * L; ++r) zq[r] = zq[r + 1];
+            asm volatile("v_mov_b32 %0, %1" : "=v"(zq[2 * L]) : "v"(f_front[d]));
+#pragma unroll
+            for (int q = 0; q < 2 * N - 1; ++q) aq[q] = aq[q + 1];
+            {
+                float dd = 0.f;  // new half point i+N-1: p(i .. i+2N-1) = zq[L .. L+2N-1]
+#pragma unroll
+                for (int m = 0; m < N; ++m) dd = fmaf(a.cz[m], zq[L + N + m] - zq[L + N - 1 - m], dd);
+                aq[2 * N - 1] = dd;
             }
-            if (xo_in) {
-                const int xs = j < nB ? j : j - (nx - nb2);
-                xps = a.xpsi[((size_t)i * ny + k) * nb2 + xs];
-                xbo = a.xbm1[(size_t)i * nb2 + xs];
-            }
-        }
-        float yph[QY], yps = 0.f;
-        bool yh_in[QY];
-        const bool yo_in = ystrip && own && live && (k < nB || k >= ny - nB);
+            float pz = 0.f;
 #pragma unroll
-        for (int u = 0; u < QY; ++u) {
-            yph[u] = 0.f;
-            const int hy = y0 - N + ty + u * TY;
-            yh_in[u] = ystrip && live && j < nx && ty + u * TY < AYH && hy >= 0 && hy < ny &&
-                       (hy < nB || hy >= ny - nB);
-            if (yh_in[u]) {
-                const int ys = hy < nB ? hy : hy - (ny - nb2);
-                yph[u] = a.yphi_in[((size_t)i * nb2 + ys) * nx + j];
-            }
-        }
-        if (yo_in) {
-            const int ys = k < nB ? k : k - (ny - nb2);
-            yps = a.ypsi[((size_t)i * nb2 + ys) * nx + j];
-        }
-        // ---- advance the z window to p(i-L .. i+L) and the A_z queue
-#pragma unroll
-        for (int r = 0; r < 2 * L; ++r) zq[r] = zq[r + 1];
-        zq[2 * L] = f_front[d];
-#pragma unroll
-        for (int q = 0; q < 2 * N - 1; ++q) aq[q] = aq[q + 1];
-        {
-            // new half point h = i+N-1: p(i .. i+2N-1) = zq[L .. L+2N-1]
-            float dd = 0.f;
-#pragma unroll
-            for (int m = 0; m < N; ++m) dd = fmaf(a.cz[m], zq[L + N + m] - zq[L + N - 1 - m], dd);
-            if (zh_in) {
-                const float phi = fmaf(zbh, zph + dd, zph);
-                dd += phi;
-                if (hz >= zbeg && hz < zend)
-                    a.zphi_out[((size_t)(hz - zb0) * ny + k) * nx + j] = phi;
-            }
-            aq[2 * N - 1] = dd;
-        }
-        const float po = f_po[d], v2 = f_v2[d];
-        float hv[HPT];
-#pragma unroll
-        for (int u = 0; u < HPT; ++u) hv[u] = f_h[d][u];
+            for (int m = 0; m < N; ++m) pz = fmaf(a.cz[m], aq[N + m] - aq[N - 1 - m], pz);
 
-        // ---- P_z = D-A_z + Psi_z
-        float pz = 0.f;
+            float *P = sp0 + buf * (PX * PY);
+            P[(ty + L) * PX + lane + L] = zq[L];
 #pragma unroll
-        for (int m = 0; m < N; ++m) pz = fmaf(a.cz[m], aq[N + m] - aq[N - 1 - m], pz);
-        if (zi_in) {
-            const float psi = fmaf(zbi, zps + pz, zps);
-            a.zpsi[((size_t)(i - zb0) * ny + k) * nx + j] = psi;
-            pz += psi;
-        }
-
-        // ---- plane i into LDS
-        float *P = sp[buf];
-        P[(ty + L) * PX + lane + L] = zq[L];
-#pragma unroll
-        for (int u = 0; u < HPT; ++u)
-            if (h_lds[u] >= 0) P[h_lds[u]] = hv[u];
-        __syncthreads();
-
-        float pxy;
-        if (!xstrip && !ystrip) {
+            for (int u = 0; u < HPT; ++u)
+                if (h_lds[u] >= 0) P[h_lds[u]] = f_h[d][u];
+            __syncthreads();
             const float *c = P + (ty + L) * PX + lane + L;
             float acc = a.w0 * c[0];
 #pragma unroll
             for (int m = 1; m <= L; ++m) acc = fmaf(a.wx[m], c[m] + c[-m], acc);
 #pragma unroll
             for (int m = 1; m <= L; ++m) acc = fmaf(a.wy[m], c[m * PX] + c[-m * PX], acc);
-            pxy = fmaf(a.w0lo, c[0], acc);  // low part of the centre weight, after cancellation
-        } else {
-            // damping strips: each direction takes the two-stage CPML form where
-            // its strip is within reach, its composite stencil otherwise
+            const float pxy = fmaf(a.w0lo, c[0], acc);  // low part of the centre weight, after cancellation
+
+            if (own && live) {
+                float s = 0.f;
+                if (a.nsrc > 0) {
+                    const int cell = (i * ny + k) * nx + j;
+                    for (int q = 0; q < a.nsrc; ++q) {
+                        const int cq = ld_uniform3(a.src_cell + q);
+                        const float amp = ld_uniform3(a.src_amp + q);
+                        if (cq == cell) s += amp;
+                    }
+                }
+                const float pc = zq[L];
+                const float pn = pc + fmaf(f_v2[d], (pz + pxy) + s, pc - f_po[d]);
+                st3_nt(pn, rO, vcol, plane(i));
+                if (a.rec && i == 0) a.rec[(size_t)k * nx + j] = pn;
+                if (a.snap) a.snap[((size_t)i * ny + k) * nx + j] = pn;
+            }
+            // refill the slot with plane i+PD; past the chunk the loads read the
+            // bottom padding (2*PD + L <= kHZbot)
+            f_front[d] = ld3(rP, vcol, plane(i + PD + L));
+            f_po[d] = ld3_nt(rO, vcol, plane(i + PD));
+            f_v2[d] = ld3_nt(rV, vcol, plane(i + PD));
+#pragma unroll
+            for (int u = 0; u < HPT; ++u) f_h[d][u] = ld3(rP, h_voff[u], plane(i + PD));
+            buf ^= 1;
+        }
+    }
+}
+
+// Sweep of a tile or chunk within reach of a damping strip: each direction
+// takes the two-stage CPML form where its strip is within reach (A_x, A_y at
+// half points through LDS; Phi ping-ponged, neighbouring tiles and chunks
+// recomputing the half points at their edges; Psi in place), its composite
+// stencil otherwise.  A plane's strip values are loaded, branch-free (pool
+// offsets, kOOB for lanes without a value), at the end of the previous plane
+// and before that plane's ring refill, so waiting for them leaves the refill
+// in flight.
+template <int N, int TY, int PD>
+__device__ __forceinline__ void cpml3d_sweep_strip(const Cpml3dArgs &a, float *sp0, float *sax,
+                                                   float *say, int tx, int tyi, int zbeg,
+                                                   int zend) {
+    using T = Tile3<N, TY>;
+    constexpr int L = T::L, PX = T::PX, PY = T::PY, HPT = T::HPT, AXW = T::AXW, AYH = T::AYH,
+                  QY = T::QY;
+    const int tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
+    const int x0 = tx * kTX, y0 = tyi * TY;
+    const int nz = a.nz, nx = a.nx, ny = a.ny, nB = a.nB;
+    const int j = x0 + lane, k = y0 + ty;
+    const bool own = j < nx && k < ny;
+    const __amdgpu_buffer_rsrc_t rP = buf_rsrc3(a.pcur, a.nbytes);
+    const __amdgpu_buffer_rsrc_t rO = buf_rsrc3(a.pold, a.nbytes);
+    const __amdgpu_buffer_rsrc_t rV = buf_rsrc3(a.vdt2, a.nbytes);
+    const __amdgpu_buffer_rsrc_t rS = buf_rsrc3(a.pool, a.pool_bytes);
+    const uint32_t vcol = ((uint32_t)(k + kHY) * a.pitch + (uint32_t)(j + kHX)) * 4u;
+    auto plane = [&](int i) -> uint32_t { return (uint32_t)(i + kHZ) * a.planeb; };
+    int h_lds[HPT];
+    uint32_t h_voff[HPT];
+    halo_slots3<N, TY>(a, tid, x0, y0, h_lds, h_voff);
+
+    const bool xstrip = x0 - N < nB || x0 + kTX + N - 1 > nx - nB;
+    const bool ystrip = y0 - N < nB || y0 + TY + N - 1 > ny - nB;
+    const bool zgroup = zbeg >= a.zsplit;  // chunk of the bottom z strip
+    const int zb0 = nz - nB;
+    const int nb2 = 2 * nB;
+    auto xsi = [&](int x) { return x < nB ? x : x - (nx - nb2); };  // x strip index
+    auto ysi = [&](int y) { return y < nB ? y : y - (ny - nb2); };  // y strip index
+    auto in_xs = [&](int x) { return x >= 0 && x < nx && (x < nB || x >= nx - nB); };
+    auto in_ys = [&](int y) { return y >= 0 && y < ny && (y < nB || y >= ny - nB); };
+    // byte offsets of this lane's strip values for plane i (kOOB: none)
+    auto o_zh = [&](int i, uint32_t base, bool cell) -> uint32_t {  // half point i+N-1
+        const int hz = i + N - 1;
+        if (!(own && hz >= zb0 && hz < nz)) return kOOB;
+        const uint32_t r = (uint32_t)(hz - zb0);
+        return base + (cell ? (r * ny + k) * nx + j : r * nx + j) * 4u;
+    };
+    auto o_zi = [&](int i, uint32_t base, bool cell) -> uint32_t {  // cell i
+        if (!(own && i < zend && i >= zb0)) return kOOB;
+        const uint32_t r = (uint32_t)(i - zb0);
+        return base + (cell ? (r * ny + k) * nx + j : r * nx + j) * 4u;
+    };
+    auto o_xh = [&](int i, int half, uint32_t base, bool cell) -> uint32_t {
+        const int hx = x0 - N + lane + half * kTX;
+        if (!(i < zend && k < ny && (half == 0 || lane < L) && in_xs(hx))) return kOOB;
+        const uint32_t xs = (uint32_t)xsi(hx);
+        return base + (cell ? ((uint32_t)i * ny + k) * nb2 + xs : (uint32_t)i * nb2 + xs) * 4u;
+    };
+    auto o_xo = [&](int i, uint32_t base, bool cell) -> uint32_t {
+        if (!(own && i < zend && in_xs(j))) return kOOB;
+        const uint32_t xs = (uint32_t)xsi(j);
+        return base + (cell ? ((uint32_t)i * ny + k) * nb2 + xs : (uint32_t)i * nb2 + xs) * 4u;
+    };
+    auto o_yh = [&](int i, int u, uint32_t base) -> uint32_t {
+        const int q = ty + u * TY, hy = y0 - N + q;
+        if (!(i < zend && j < nx && q < AYH && in_ys(hy))) return kOOB;
+        return base + (((uint32_t)i * nb2 + ysi(hy)) * nx + j) * 4u;
+    };
+    auto o_yo = [&](int i, uint32_t base) -> uint32_t {
+        if (!(own && i < zend && in_ys(k))) return kOOB;
+        return base + (((uint32_t)i * nb2 + ysi(k)) * nx + j) * 4u;
+    };
+
+    // this plane's strip values (loaded during the previous plane)
+    float zph, zbh, zps, zbi, xph[2], xbh[2], xps, xbo, yph[QY], yps;
+    auto load_strip = [&](int i) {
+        if (zgroup) {
+            zph = ldS(rS, o_zh(i, a.o_zphi_in, true));
+            zbh = ldS(rS, o_zh(i, a.o_zbm1, false));
+            zps = ldS(rS, o_zi(i, a.o_zpsi, true));
+            zbi = ldS(rS, o_zi(i, a.o_zbm1, false));
+        }
+        if (xstrip) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                xph[half] = ldS(rS, o_xh(i, half, a.o_xphi_in, true));
+                xbh[half] = ldS(rS, o_xh(i, half, a.o_xbm1, false));
+            }
+            xps = ldS(rS, o_xo(i, a.o_xpsi, true));
+            xbo = ldS(rS, o_xo(i, a.o_xbm1, false));
+        }
+        if (ystrip) {
+#pragma unroll
+            for (int u = 0; u < QY; ++u) yph[u] = ldS(rS, o_yh(i, u, a.o_yphi_in));
+            yps = ldS(rS, o_yo(i, a.o_ypsi));
+        }
+    };
+    // y-strip b-1 values do not change along z: loaded once per block
+    float ybh[QY], ybo = 0.f;
+    if (ystrip) {
+#pragma unroll
+        for (int u = 0; u < QY; ++u) {
+            const int q = ty + u * TY, hy = y0 - N + q;
+            ybh[u] = ldS(rS, j < nx && q < AYH && in_ys(hy)
+                                 ? a.o_ybm1 + ((uint32_t)ysi(hy) * nx + j) * 4u : kOOB);
+        }
+        ybo = ldS(rS, own && in_ys(k) ? a.o_ybm1 + ((uint32_t)ysi(k) * nx + j) * 4u : kOOB);
+    }
+
+    // z window p(i-L .. i+L) and A_z queue (half points i-N .. i+N-1)
+    float zq[2 * L + 1], aq[2 * N];
+#pragma unroll
+    for (int r = 0; r < 2 * L; ++r) zq[r + 1] = ld3(rP, vcol, plane(zbeg - L + r));
+#pragma unroll
+    for (int q = 0; q < 2 * N - 1; ++q) {
+        // half point h = zbeg-N+q uses p(h-N+1 .. h+N) = zq[1+q .. 1+q+2N-1]
+        float d = 0.f;
+#pragma unroll
+        for (int m = 0; m < N; ++m) d = fmaf(a.cz[m], zq[1 + q + N + m] - zq[q + N - m], d);
+        if (zgroup) {
+            const int h = zbeg - N + q;
+            const bool ok = own && h >= zb0 && h < nz;
+            const uint32_t o = ok ? (((uint32_t)(h - zb0) * ny + k) * nx + j) * 4u : kOOB;
+            const float ph = ldS(rS, ok ? a.o_zphi_in + o : kOOB);
+            const float bh = ldS(rS, ok ? a.o_zbm1 + ((uint32_t)(h - zb0) * nx + j) * 4u : kOOB);
+            const float phi = fmaf(bh, ph + d, ph);
+            d += phi;
+            stS(phi, rS, ok && h >= zbeg && h < zend ? a.o_zphi_out + o : kOOB);
+        }
+        aq[q + 1] = d;
+    }
+
+    // strip values of the first plane, then the prefetch ring in slot order
+    load_strip(zbeg);
+    float f_front[PD], f_po[PD], f_v2[PD], f_h[PD][HPT];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int d = 0; d < PD; ++d) {
+        const int ip = min(zbeg + d, nz - 1);  // in bounds; unused past zend
+        f_po[d] = ld3_nt(rO, vcol, plane(ip));
+        f_v2[d] = ld3_nt(rV, vcol, plane(ip));
+        f_front[d] = ld3(rP, vcol, plane(ip + L));
+#pragma unroll
+        for (int u = 0; u < HPT; ++u) f_h[d][u] = ld3(rP, h_voff[u], plane(ip));
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    int buf = 0;
+    for (int i0 = zbeg; i0 < zend; i0 += PD) {
+#pragma unroll
+        for (int d = 0; d < PD; ++d) {
+            const int i = i0 + d;
+            const bool live = i < zend;
+#pragma unroll
+            for (int r = 0; r < 2 * L; ++r) zq[r] = zq[r + 1];
+            asm volatile("v_mov_b32 %0, %1" : "=v"(zq[2 * L]) : "v"(f_front[d]));
+#pragma unroll
+            for (int q = 0; q < 2 * N - 1; ++q) aq[q] = aq[q + 1];
+
+            // plane i into LDS
+            float *P = sp0 + buf * (PX * PY);
+            P[(ty + L) * PX + lane + L] = zq[L];
+#pragma unroll
+            for (int u = 0; u < HPT; ++u)
+                if (h_lds[u] >= 0) P[h_lds[u]] = f_h[d][u];
+            __syncthreads();
+
+            // z: new A_z half point i+N-1 from p(i .. i+2N-1) = zq[L .. L+2N-1],
+            // then P_z = D-A_z + Psi_z
+            {
+                float dd = 0.f;
+#pragma unroll
+                for (int m = 0; m < N; ++m) dd = fmaf(a.cz[m], zq[L + N + m] - zq[L + N - 1 - m], dd);
+                if (zgroup) {
+                    const float phi = fmaf(zbh, zph + dd, zph);
+                    dd += phi;
+                    const int hz = i + N - 1;
+                    stS(phi, rS, hz >= zbeg && hz < zend ? o_zh(i, a.o_zphi_out, true) : kOOB);
+                }
+                aq[2 * N - 1] = dd;
+            }
+            float pz = 0.f;
+#pragma unroll
+            for (int m = 0; m < N; ++m) pz = fmaf(a.cz[m], aq[N + m] - aq[N - 1 - m], pz);
+            if (zgroup) {
+                const float psi = fmaf(zbi, zps + pz, zps);
+                stS(psi, rS, o_zi(i, a.o_zpsi, true));
+                pz += psi;
+            }
+
+            const float *c = P + (ty + L) * PX + lane + L;
+            float pxy;
+            if (!xstrip && !ystrip) {
+                // a z-strip chunk of an interior column tile: composite x and y
+                float acc = a.w0 * c[0];
+#pragma unroll
+                for (int m = 1; m <= L; ++m) acc = fmaf(a.wx[m], c[m] + c[-m], acc);
+#pragma unroll
+                for (int m = 1; m <= L; ++m) acc = fmaf(a.wy[m], c[m * PX] + c[-m * PX], acc);
+                pxy = fmaf(a.w0lo, c[0], acc);
+            } else {
+            float px, py;
             if (xstrip) {
                 // A_x at half points hx = x0-N+q (q < 64+L) for row ty
                 const float *row = P + (ty + L) * PX;
@@ -332,19 +490,14 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
                 for (int half = 0; half < 2; ++half) {
                     const int q = lane + half * kTX;
                     if (half == 1 && lane >= L) break;
-                    float d = 0.f;
+                    float dx = 0.f;
 #pragma unroll
-                    for (int m = 0; m < N; ++m) d = fmaf(a.cx[m], row[q + N + m] - row[q + N - 1 - m], d);
+                    for (int m = 0; m < N; ++m) dx = fmaf(a.cx[m], row[q + N + m] - row[q + N - 1 - m], dx);
+                    const float phi = fmaf(xbh[half], xph[half] + dx, xph[half]);
+                    dx += phi;
                     const int hx = x0 - N + q;
-                    if (xh_in[half]) {
-                        const float phi = fmaf(xbh[half], xph[half] + d, xph[half]);
-                        d += phi;
-                        if (hx >= x0 && hx < x0 + kTX) {
-                            const int xs = hx < nB ? hx : hx - (nx - nb2);
-                            a.xphi_out[((size_t)i * ny + k) * nb2 + xs] = phi;
-                        }
-                    }
-                    sax[ty * AXW + q] = d;
+                    stS(phi, rS, hx >= x0 && hx < x0 + kTX ? o_xh(i, half, a.o_xphi_out, true) : kOOB);
+                    sax[ty * AXW + q] = dx;
                 }
             }
             if (ystrip) {
@@ -354,20 +507,15 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
                     const int q = ty + u * TY;
                     if (q >= AYH) break;
                     const float *cl = P + lane + L;
-                    float d = 0.f;
+                    float dy = 0.f;
 #pragma unroll
                     for (int m = 0; m < N; ++m)
-                        d = fmaf(a.cy[m], cl[(q + N + m) * PX] - cl[(q + N - 1 - m) * PX], d);
+                        dy = fmaf(a.cy[m], cl[(q + N + m) * PX] - cl[(q + N - 1 - m) * PX], dy);
+                    const float phi = fmaf(ybh[u], yph[u] + dy, yph[u]);
+                    dy += phi;
                     const int hy = y0 - N + q;
-                    if (yh_in[u]) {
-                        const float phi = fmaf(ybh[u], yph[u] + d, yph[u]);
-                        d += phi;
-                        if (hy >= y0 && hy < y0 + TY) {
-                            const int ys = hy < nB ? hy : hy - (ny - nb2);
-                            a.yphi_out[((size_t)i * nb2 + ys) * nx + j] = phi;
-                        }
-                    }
-                    say[q * kTX + lane] = d;
+                    stS(phi, rS, hy >= y0 && hy < y0 + TY ? o_yh(i, u, a.o_yphi_out) : kOOB);
+                    say[q * kTX + lane] = dy;
                 }
             }
             // A_x rows are written and read by the same wavefront (row ty); A_y
@@ -379,19 +527,14 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            const float *c = P + (ty + L) * PX + lane + L;
-            float px, py;
             if (xstrip) {
                 px = 0.f;
                 const float *ax = sax + ty * AXW + lane + N;
 #pragma unroll
                 for (int m = 0; m < N; ++m) px = fmaf(a.cx[m], ax[m] - ax[-1 - m], px);
-                if (xo_in) {
-                    const int xs = j < nB ? j : j - (nx - nb2);
-                    const float psi = fmaf(xbo, xps + px, xps);
-                    a.xpsi[((size_t)i * ny + k) * nb2 + xs] = psi;
-                    px += psi;
-                }
+                const float psi = fmaf(xbo, xps + px, xps);
+                stS(psi, rS, o_xo(i, a.o_xpsi, true));
+                px += psi;
             } else {
                 float acc = a.w0x * c[0];
 #pragma unroll
@@ -403,12 +546,9 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
                 const float *ay = say + (ty + N) * kTX + lane;
 #pragma unroll
                 for (int m = 0; m < N; ++m) py = fmaf(a.cy[m], ay[m * kTX] - ay[(-1 - m) * kTX], py);
-                if (yo_in) {
-                    const int ys = k < nB ? k : k - (ny - nb2);
-                    const float psi = fmaf(ybo, yps + py, yps);
-                    a.ypsi[((size_t)i * nb2 + ys) * nx + j] = psi;
-                    py += psi;
-                }
+                const float psi = fmaf(ybo, yps + py, yps);
+                stS(psi, rS, o_yo(i, a.o_ypsi));
+                py += psi;
             } else {
                 float acc = a.w0y * c[0];
 #pragma unroll
@@ -416,38 +556,73 @@ __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
                 py = fmaf(a.w0ylo, c[0], acc);
             }
             pxy = px + py;
-        }
-
-        // ---- update
-        if (own && live) {
-            float s = 0.f;
-            if (a.nsrc > 0) {
-                const int cell = (i * ny + k) * nx + j;
-                for (int q = 0; q < a.nsrc; ++q) {
-                    const int c = ld_uniform3(a.src_cell + q);
-                    const float amp = ld_uniform3(a.src_amp + q);
-                    if (c == cell) s += amp;
-                }
             }
-            const float pc = zq[L];
-            const float pn = pc + fmaf(v2, (pz + pxy) + s, pc - po);
-            st3_nt(pn, rO, vcol, plane(i));
-            if (a.rec && i == 0) a.rec[(size_t)k * nx + j] = pn;
-            if (a.snap) a.snap[((size_t)i * ny + k) * nx + j] = pn;
-        }
-        // ---- refill the slot with plane i+PD once its old contents are used
-        // (the loads then reuse the slot's registers and stay in flight across
-        // the loop back-edge); past the chunk they read the bottom padding
-        // (2*PD + L <= kHZbot)
-        f_front[d] = ld3(rP, vcol, plane(i + PD + L));
-        f_po[d] = ld3_nt(rO, vcol, plane(i + PD));
-        f_v2[d] = ld3_nt(rV, vcol, plane(i + PD));
+
+            if (own && live) {
+                float s = 0.f;
+                if (a.nsrc > 0) {
+                    const int cell = (i * ny + k) * nx + j;
+                    for (int q = 0; q < a.nsrc; ++q) {
+                        const int cq = ld_uniform3(a.src_cell + q);
+                        const float amp = ld_uniform3(a.src_amp + q);
+                        if (cq == cell) s += amp;
+                    }
+                }
+                const float pc = zq[L];
+                const float pn = pc + fmaf(f_v2[d], (pz + pxy) + s, pc - f_po[d]);
+                st3_nt(pn, rO, vcol, plane(i));
+                if (a.rec && i == 0) a.rec[(size_t)k * nx + j] = pn;
+                if (a.snap) a.snap[((size_t)i * ny + k) * nx + j] = pn;
+            }
+            // next plane's strip values, then the ring refill with plane i+PD
+            load_strip(i + 1);
+            f_front[d] = ld3(rP, vcol, plane(i + PD + L));
+            f_po[d] = ld3_nt(rO, vcol, plane(i + PD));
+            f_v2[d] = ld3_nt(rV, vcol, plane(i + PD));
 #pragma unroll
-        for (int u = 0; u < HPT; ++u) f_h[d][u] = ld3(rP, h_voff[u], plane(i + PD));
-        buf ^= 1;
+            for (int u = 0; u < HPT; ++u) f_h[d][u] = ld3(rP, h_voff[u], plane(i + PD));
+            buf ^= 1;
         }
     }
+}
+
+// Chunk tz of the z split: chunks of zc planes over [0, zsplit), then over
+// [zsplit, nz).  Host and device share it.
+__host__ __device__ inline void chunk_bounds3(int tz, int zc, int zsplit, int nzA, int nz,
+                                              int *zbeg, int *zend) {
+    if (tz < nzA) {
+        *zbeg = tz * zc;
+        *zend = *zbeg + zc < zsplit ? *zbeg + zc : zsplit;
+    } else {
+        *zbeg = zsplit + (tz - nzA) * zc;
+        *zend = *zbeg + zc < nz ? *zbeg + zc : nz;
     }
+}
+
+// One time step.  Each block decodes its (x tile, y tile, z chunk) from the
+// launch order and runs the strip or the interior sweep (a block-uniform
+// branch: the two loops are compiled apart, so the interior loop carries none
+// of the strip code's loads).
+template <int N, int TY, int PD = kPD3>
+__global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
+    using T = Tile3<N, TY>;
+    __shared__ float sp[2 * T::PX * T::PY];
+    __shared__ float sax[TY * T::AXW];
+    __shared__ float say[T::AYH * kTX];
+    int b = a.order ? ld_uniform3(a.order + blockIdx.x) : (int)blockIdx.x;
+    const int tx = b % a.ntx;
+    b /= a.ntx;
+    const int tyi = b % a.nty;
+    const int tz = b / a.nty;
+    int zbeg, zend;
+    chunk_bounds3(tz, a.zc, a.zsplit, a.nzA, a.nz, &zbeg, &zend);
+    const int x0 = tx * kTX, y0 = tyi * TY, nB = a.nB;
+    const bool strip = nB > 0 && (x0 - N < nB || x0 + kTX + N - 1 > a.nx - nB ||
+                                  y0 - N < nB || y0 + TY + N - 1 > a.ny - nB || zbeg >= a.zsplit);
+    if (strip)
+        cpml3d_sweep_strip<N, TY, PD>(a, sp, sax, say, tx, tyi, zbeg, zend);
+    else
+        cpml3d_sweep_inner<N, TY, PD>(a, sp, tx, tyi, zbeg, zend);
 }
 
 // ---------------------------------------------------------------------------
@@ -468,7 +643,10 @@ class Prop3D final : public Prop {
    public:
     int64_t pitch = 0, npy = 0, nzp = 0;
     DevBuf<float> p[2], vfield;
-    DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1, yphi[2], ypsi, ybm1;
+    // damping strips: one pool; b-1 tables first, then the Phi/Psi state
+    DevBuf<float> pool;
+    size_t o_zbm1 = 0, o_xbm1 = 0, o_ybm1 = 0, o_zphi[2] = {0, 0}, o_zpsi = 0, o_xphi[2] = {0, 0},
+           o_xpsi = 0, o_yphi[2] = {0, 0}, o_ypsi = 0, o_state = 0;  // in floats
     int cur = 0, phase = 0;
     float cz[8] = {0}, cx[8] = {0}, cy[8] = {0}, w0 = 0.f, w0lo = 0.f, wx[16] = {0}, wy[16] = {0};
     float w0x = 0.f, w0xlo = 0.f, w0y = 0.f, w0ylo = 0.f;
@@ -562,22 +740,29 @@ class Prop3D final : public Prop {
             for (int q = 0; q < nB; ++q)
                 for (int64_t j = 0; j < nx; ++j)
                     hz[(size_t)q * nx + j] = bm1(V(nz - nB + q, j, 0), (double)(q + 1));
-            xbm1.alloc(hx.size());
-            ybm1.alloc(hy.size());
-            zbm1.alloc(hz.size());
-            SSSI_HIP(hipMemcpy(xbm1.p, hx.data(), xbm1.bytes(), hipMemcpyHostToDevice));
-            SSSI_HIP(hipMemcpy(ybm1.p, hy.data(), ybm1.bytes(), hipMemcpyHostToDevice));
-            SSSI_HIP(hipMemcpy(zbm1.p, hz.data(), zbm1.bytes(), hipMemcpyHostToDevice));
             const size_t zn = (size_t)nB * ny * nx, xn = (size_t)nz * ny * 2 * nB,
                          yn = (size_t)nz * 2 * nB * nx;
+            size_t top = 0;
+            auto take = [&](size_t n) { const size_t o = top; top += round_up((int64_t)n, 64); return o; };
+            o_zbm1 = take(hz.size());
+            o_xbm1 = take(hx.size());
+            o_ybm1 = take(hy.size());
+            o_state = top;
             for (int q = 0; q < 2; ++q) {
-                zphi[q].alloc(zn);
-                xphi[q].alloc(xn);
-                yphi[q].alloc(yn);
+                o_zphi[q] = take(zn);
+                o_xphi[q] = take(xn);
+                o_yphi[q] = take(yn);
             }
-            zpsi.alloc(zn);
-            xpsi.alloc(xn);
-            ypsi.alloc(yn);
+            o_zpsi = take(zn);
+            o_xpsi = take(xn);
+            o_ypsi = take(yn);
+            // byte offsets are 32-bit and must stay below the out-of-range marker
+            if (top * 4 >= ((size_t)1 << 31))
+                fail(SSSI_EINVAL, "3-D damping-strip state exceeds 2 GiB (nBoundary %d)", nB);
+            pool.alloc(top);
+            SSSI_HIP(hipMemcpy(pool.p + o_zbm1, hz.data(), hz.size() * 4, hipMemcpyHostToDevice));
+            SSSI_HIP(hipMemcpy(pool.p + o_xbm1, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
+            SSSI_HIP(hipMemcpy(pool.p + o_ybm1, hy.data(), hy.size() * 4, hipMemcpyHostToDevice));
         }
         reset();
         if (autotune_enabled()) {
@@ -648,9 +833,9 @@ class Prop3D final : public Prop {
     int64_t cells() const override { return nz * nx * ny; }
 
     void reset() override {
-        for (auto *b : {&p[0], &p[1], &zphi[0], &zphi[1], &zpsi, &xphi[0], &xphi[1], &xpsi,
-                        &yphi[0], &yphi[1], &ypsi})
-            if (b->p) SSSI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), stream));
+        for (auto *b : {&p[0], &p[1]}) SSSI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), stream));
+        if (pool.p)
+            SSSI_HIP(hipMemsetAsync(pool.p + o_state, 0, (pool.n - o_state) * sizeof(float), stream));
         cur = 0;
         phase = 0;
         SSSI_HIP(hipStreamSynchronize(stream));
@@ -690,18 +875,28 @@ class Prop3D final : public Prop {
         hipLaunchKernelGGL((cpml3d_step<NN, TY>), dim3(nblk), dim3(64 * TY), 0, stream, args);
     }
 
-    // Blocks whose x or y stencil reach touches a damping strip go first (they
-    // run the two-stage form and carry the strip arrays' traffic); the others
-    // follow in 8 contiguous bands, block b landing on XCD b % 8.
+    // z split of the sweep: chunks above zsplit never reach the bottom strip
+    int zsplit() const { return nB > 0 ? (int)std::max<int64_t>(0, nz - nB - N + 1) : (int)nz; }
+    int nzA() const { return (zsplit() + zc - 1) / zc; }
+    int nzc() const { return nzA() + (int)((nz - zsplit() + zc - 1) / zc); }
+
+    // Blocks whose stencil reach touches a damping strip (x, y, or a chunk of
+    // the bottom z strip) go first (they run the two-stage form and carry the
+    // strip arrays' traffic); the others follow in 8 contiguous bands, block b
+    // landing on XCD b % 8.
     void build_order(int ntx, int nty, int nzc) {
         const int nblk = ntx * nty * nzc;
+        const int zs = zsplit(), na = nzA();
         std::vector<int> strip, inner;
         for (int b = 0; b < nblk; ++b) {
-            const int tx = b % ntx, tyi = (b / ntx) % nty;
+            const int tx = b % ntx, tyi = (b / ntx) % nty, tz = b / (ntx * nty);
             const int64_t x0 = (int64_t)tx * kTX, y0 = (int64_t)tyi * ty;
+            int zbeg, zend;
+            chunk_bounds3(tz, zc, zs, na, (int)nz, &zbeg, &zend);
             const bool xs = nB > 0 && (x0 - N < nB || x0 + kTX + N - 1 > nx - nB);
             const bool ys = nB > 0 && (y0 - N < nB || y0 + ty + N - 1 > ny - nB);
-            (xs || ys ? strip : inner).push_back(b);
+            const bool zs_ = nB > 0 && zbeg >= zs;
+            (xs || ys || zs_ ? strip : inner).push_back(b);
         }
         std::vector<int> ord(strip);
         const size_t n = inner.size(), base = ord.size();
@@ -734,6 +929,8 @@ class Prop3D final : public Prop {
         a.ntx = (int)((nx + kTX - 1) / kTX);
         a.nty = (int)((ny + ty - 1) / ty);
         a.zc = zc;
+        a.zsplit = zsplit();
+        a.nzA = nzA();
         a.rowb = (uint32_t)pitch * 4u;
         a.planeb = (uint32_t)(pitch * npy) * 4u;
         a.nbytes = (uint32_t)(p[0].bytes());
@@ -746,9 +943,15 @@ class Prop3D final : public Prop {
         a.w0ylo = w0ylo;
         for (int m = 0; m < 16; ++m) { a.wx[m] = wx[m]; a.wy[m] = wy[m]; }
         if (nB > 0) {
-            a.zphi_in = zphi[phase].p; a.zphi_out = zphi[phase ^ 1].p; a.zpsi = zpsi.p; a.zbm1 = zbm1.p;
-            a.xphi_in = xphi[phase].p; a.xphi_out = xphi[phase ^ 1].p; a.xpsi = xpsi.p; a.xbm1 = xbm1.p;
-            a.yphi_in = yphi[phase].p; a.yphi_out = yphi[phase ^ 1].p; a.ypsi = ypsi.p; a.ybm1 = ybm1.p;
+            auto ob = [](size_t o) { return (uint32_t)(o * 4); };
+            a.pool = pool.p;
+            a.pool_bytes = (uint32_t)pool.bytes();
+            a.o_zphi_in = ob(o_zphi[phase]); a.o_zphi_out = ob(o_zphi[phase ^ 1]);
+            a.o_zpsi = ob(o_zpsi); a.o_zbm1 = ob(o_zbm1);
+            a.o_xphi_in = ob(o_xphi[phase]); a.o_xphi_out = ob(o_xphi[phase ^ 1]);
+            a.o_xpsi = ob(o_xpsi); a.o_xbm1 = ob(o_xbm1);
+            a.o_yphi_in = ob(o_yphi[phase]); a.o_yphi_out = ob(o_yphi[phase ^ 1]);
+            a.o_ypsi = ob(o_ypsi); a.o_ybm1 = ob(o_ybm1);
         }
         if (nsrc > 0 && t < src_nt) {
             a.src_cell = src_cell.p;
@@ -757,9 +960,8 @@ class Prop3D final : public Prop {
         }
         if (record) a.rec = rec.p + (size_t)t * nx * ny;
         a.snap = snap;
-        const int nzc = (int)((nz + zc - 1) / zc);
-        const int nblk = a.ntx * a.nty * nzc;
-        if (order_ty != ty || order_zc != zc) build_order(a.ntx, a.nty, nzc);
+        const int nblk = a.ntx * a.nty * nzc();
+        if (order_ty != ty || order_zc != zc) build_order(a.ntx, a.nty, nzc());
         a.order = order.p;
         switch (N * 100 + ty) {
 #define SSSI_C3(NN, T) case NN * 100 + T: launch<NN, T>(a, nblk); break;
