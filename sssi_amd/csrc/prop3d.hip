@@ -95,8 +95,8 @@ __device__ __forceinline__ void st3(float v, __amdgpu_buffer_rsrc_t r, uint32_t 
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
 }
 
-constexpr int kPD3 = 2;  // planes prefetched ahead by the 3-D sweep
-static_assert(2 * kPD3 + 15 <= kHZbot, "bottom padding must hold the sweep's overrun");
+constexpr int kPD3max = 4;  // deepest prefetch ring of the 3-D sweep (planes)
+static_assert(2 * kPD3max + 15 <= kHZbot, "bottom padding must hold the sweep's overrun");
 
 // Strip state lives in one pool behind one buffer resource; a lane without a
 // value in a strip array takes this byte offset, past the end of any pool
@@ -603,8 +603,9 @@ __host__ __device__ inline void chunk_bounds3(int tz, int zc, int zsplit, int nz
 // launch order and runs the strip or the interior sweep (a block-uniform
 // branch: the two loops are compiled apart, so the interior loop carries none
 // of the strip code's loads).
-template <int N, int TY, int PD = kPD3>
+template <int N, int TY, int PD>
 __global__ void __launch_bounds__(64 * TY) cpml3d_step(Cpml3dArgs a) {
+    static_assert(PD >= 1 && PD <= kPD3max, "prefetch ring depth");
     using T = Tile3<N, TY>;
     __shared__ float sp[2 * T::PX * T::PY];
     __shared__ float sax[TY * T::AXW];
@@ -650,7 +651,7 @@ class Prop3D final : public Prop {
     int cur = 0, phase = 0;
     float cz[8] = {0}, cx[8] = {0}, cy[8] = {0}, w0 = 0.f, w0lo = 0.f, wx[16] = {0}, wy[16] = {0};
     float w0x = 0.f, w0xlo = 0.f, w0y = 0.f, w0ylo = 0.f;
-    int ty = 8, zc = 64;
+    int ty = 8, zc = 32;
     DevBuf<int> order;  // block launch order for (order_ty, order_zc)
     int order_ty = 0, order_zc = 0;
 
@@ -872,7 +873,9 @@ class Prop3D final : public Prop {
 
     template <int NN, int TY>
     void launch(const Cpml3dArgs &args, int nblk) {
-        hipLaunchKernelGGL((cpml3d_step<NN, TY>), dim3(nblk), dim3(64 * TY), 0, stream, args);
+        // ring depth: 4 planes where the registers allow it at occupancy 4
+        constexpr int PD = TY >= 8 && NN <= 4 ? 4 : 2;
+        hipLaunchKernelGGL((cpml3d_step<NN, TY, PD>), dim3(nblk), dim3(64 * TY), 0, stream, args);
     }
 
     // z split of the sweep: chunks above zsplit never reach the bottom strip
