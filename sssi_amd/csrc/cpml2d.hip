@@ -237,11 +237,14 @@ __device__ __forceinline__ void interior_lap(const Job2D &J, const float (&win)[
 }
 
 // x halo of RZ rows (L columns each side of the 64-wide tile): lane e of
-// load u fetches value e + 64u; h_lds is its LDS slot (or -1).
+// load u fetches value e + 64u; h_lds is its LDS slot, or for lanes without a
+// value the lane's word of the wave's spare LDS row (index dummy + lane), so
+// the LDS write is unconditional and the compiler cannot sink the load into a
+// branch, where waiting for it would drain every load issued before.
 template <int N, int RZ>
 __device__ __forceinline__ void halo_slots(int lane, int (&h_lds)[(2 * (2 * N - 1) * RZ + 63) / 64],
                                            uint32_t (&h_off)[(2 * (2 * N - 1) * RZ + 63) / 64],
-                                           int x0, int pitch) {
+                                           int x0, int pitch, int dummy) {
     constexpr int L = 2 * N - 1;
     constexpr int PW = kTX + 2 * L;
     constexpr int HN = 2 * L * RZ;
@@ -251,9 +254,20 @@ __device__ __forceinline__ void halo_slots(int lane, int (&h_lds)[(2 * (2 * N - 
         const int e = lane + 64 * u;
         const int r = e / (2 * L), c = e - r * (2 * L);
         const int xc = c < L ? x0 - L + c : x0 + kTX - L + c;
-        h_lds[u] = e < HN ? r * PW + (c < L ? c : kTX + c) : -1;
-        h_off[u] = (uint32_t)(r * pitch + xc + kHX) * 4u;
+        h_lds[u] = e < HN ? r * PW + (c < L ? c : kTX + c) : dummy + lane;
+        h_off[u] = e < HN ? (uint32_t)(r * pitch + xc + kHX) * 4u : (uint32_t)(x0 + kHX) * 4u;
     }
+}
+
+// LDS of a 4-wave block: RZ rows of PW floats per wave, then one spare row of
+// 64 floats per wave (halo_slots' sink for lanes without a halo value).
+template <int N, int RZ>
+constexpr int lds_p_floats() { return 4 * RZ * (kTX + 2 * (2 * N - 1)) + 4 * kTX; }
+// index of wave w's spare row relative to its rows at lds_p + w * RZ * PW
+template <int N, int RZ>
+__device__ __forceinline__ int spare_row(int w) {
+    constexpr int PW = kTX + 2 * (2 * N - 1);
+    return (4 - w) * RZ * PW + w * kTX;
 }
 
 template <int N, int RZ>
@@ -283,8 +297,15 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
     const uint32_t vcol = (uint32_t)(j + kHX) * 4u;
     const uint32_t srow = (uint32_t)(zs + kHZ) * rowb;  // first row of the tile
 
-    // ---- issue every global load of the tile
+    // ---- issue every global load of the tile: the x halo first (vmcnt counts
+    // in issue order, so its LDS exchange then waits for it alone), then the
+    // z window, then p_old and v^2dt^2
     float col[RZ + 2 * L], po[RZ], v2[RZ], hal[HPL];
+    int h_lds[HPL];
+    uint32_t h_off[HPL];
+    halo_slots<N, RZ>(lane, h_lds, h_off, x0, J.pitch, spare_row<N, RZ>(w));
+#pragma unroll
+    for (int u = 0; u < HPL; ++u) hal[u] = buf_ld(rP, h_off[u], srow);
 #pragma unroll
     for (int r = 0; r < RZ + 2 * L; ++r) col[r] = buf_ld(rP, vcol, srow + (r - L) * rowb);
 #pragma unroll
@@ -292,19 +313,13 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
         po[r] = buf_ld(rO, vcol, srow + r * rowb);
         v2[r] = buf_ld(rV, vcol, srow + r * rowb);
     }
-    int h_lds[HPL];
-    uint32_t h_off[HPL];
-    halo_slots<N, RZ>(lane, h_lds, h_off, x0, J.pitch);
-#pragma unroll
-    for (int u = 0; u < HPL; ++u) hal[u] = buf_ld(rP, h_off[u], srow);
 
     // ---- the tile's rows with an L-wide x halo into the wave's LDS rows
     float *sp = lds_p + w * (RZ * PW);
 #pragma unroll
-    for (int r = 0; r < RZ; ++r) sp[r * PW + L + lane] = col[L + r];
+    for (int u = 0; u < HPL; ++u) sp[h_lds[u]] = hal[u];
 #pragma unroll
-    for (int u = 0; u < HPL; ++u)
-        if (h_lds[u] >= 0) sp[h_lds[u]] = hal[u];
+    for (int r = 0; r < RZ; ++r) sp[r * PW + L + lane] = col[L + r];
     wave_lds_sync();
 
     const int zb0 = nz - nB;
@@ -509,7 +524,7 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
     const uint32_t srow0 = (uint32_t)(z0 + kHZ) * rowb;
     int h_lds[HPL];
     uint32_t h_off[HPL];
-    halo_slots<N, RZ>(lane, h_lds, h_off, x0, J.pitch);
+    halo_slots<N, RZ>(lane, h_lds, h_off, x0, J.pitch, spare_row<N, RZ>(w));
 
     float win[WN], po[ZB], v2[ZB], hal[NG][HPL];
     float *sp = lds_p + w * (RZ * PW);
@@ -541,8 +556,7 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
 #pragma unroll
         for (int r = 0; r < RZ; ++r) sp[r * PW + L + lane] = win[g * RZ + L + r];
 #pragma unroll
-        for (int u = 0; u < HPL; ++u)
-            if (h_lds[u] >= 0) sp[h_lds[u]] = hal[g][u];
+        for (int u = 0; u < HPL; ++u) sp[h_lds[u]] = hal[g][u];
         wave_lds_sync();
         float lap[RZ], s[RZ], pc[RZ], pog[RZ], v2g[RZ];
         interior_lap<N, RZ>(J, win, g * RZ, sp, lane, lap);
@@ -564,7 +578,7 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
 // interior march blocks (-1 - index).
 template <int N, int RZ, int NG>
 __device__ __forceinline__ void cpml2d_block(const Job2D &J, int tile, int step) {
-    __shared__ float s_p[4 * RZ * (kTX + 2 * (2 * N - 1))];
+    __shared__ float s_p[lds_p_floats<N, RZ>()];
     __shared__ float s_a[4 * RZ * (kTX + 2 * N - 1)];
     if constexpr (NG > 0) {
         if (tile < 0) {
@@ -581,7 +595,7 @@ template <int N, int RZ, int NG>
 __global__ void __launch_bounds__(256) cpml2d_single(Job2D J, const int2 *__restrict__ order,
                                                      int step, const int *__restrict__ base) {
     if (base) step += ld_uniform(base);
-    cpml2d_block<N, RZ, NG>(J, order[blockIdx.x].y, step);
+    cpml2d_block<N, RZ, NG>(J, ld_uniform(&order[blockIdx.x].y), step);
 }
 
 template <int N, int RZ, int NG>
@@ -589,8 +603,8 @@ __global__ void __launch_bounds__(256) cpml2d_batch(const Job2D *__restrict__ jo
                                                     const int2 *__restrict__ order, int step,
                                                     const int *__restrict__ base) {
     if (base) step += ld_uniform(base);
-    const int2 jt = order[blockIdx.x];
-    cpml2d_block<N, RZ, NG>(jobs[jt.x], jt.y, step);
+    const int jx = ld_uniform(&order[blockIdx.x].x), jy = ld_uniform(&order[blockIdx.x].y);
+    cpml2d_block<N, RZ, NG>(jobs[jx], jy, step);
 }
 
 __global__ void step_base_set(int *base, int v) {
