@@ -64,3 +64,27 @@ def test_fwdTimeCpmlFor3dAw_dense_signature():
     assert d_gpu.shape == (nx, ny, nt) and s_gpu.shape == (nz, nx, ny, nt)
     assert rel_l2(d_gpu, d_ref) < TOL
     assert rel_l2(s_gpu, s_ref) < TOL
+
+
+@pytest.mark.parametrize("nz,nx,ny,nB,N,ty,zc", [
+    (24, 70, 20, 0, 3, 8, 16),   # no damping strips: every block takes the interior sweep
+    (10, 40, 24, 8, 4, 8, 8),    # z split at depth 0: every chunk reaches the bottom strip
+    (40, 72, 26, 6, 4, 8, 8),    # chunks of 7 and 1 planes at the split and the bottom, 4-plane ring
+    (40, 72, 40, 6, 4, 16, 12),  # 16-row tiles with the 4-plane ring
+])
+def test_fwd_cpml_3d_chunking_edge_cases(nz, nx, ny, nB, N, ty, zc, kernel_env):
+    os.environ["SSSI_TY3"] = str(ty)
+    os.environ["SSSI_ZC3"] = str(zc)
+    nt = 90
+    v = synth.cake_3d_dipping(nz, nx, ny, 3, 1500.0, 3000.0, seed=11 + nz)
+    dx = dy = dz = 10.0
+    dt = 0.4 * dx / 3000.0 / np.sqrt(3) / sum(abs(c) for c in synth.dCoef(N))
+    w = synth.ricker(25, nt, dt)
+    iz, ix, iy = [0, nz // 2], [nx // 2, 5], [ny // 2, ny - 4]
+    ww = np.stack([w, 0.5 * w], axis=1)
+    d_ref, s_ref = oracle.fwd_cpml_3d(v, nt, N, nB, dz, dx, dy, dt, src_iz=iz, src_ix=ix,
+                                      src_iy=iy, wavelet=ww, want_snapshot=True, snap_stride=11)
+    d_gpu, s_gpu = sssi_amd.fwd_cpml_3d_points(v, nt, N, nB, dz, dx, dy, dt, iz, ix, iy, ww,
+                                               want_snapshot=True, snap_stride=11)
+    assert rel_l2(d_gpu, d_ref) < TOL, rel_l2(d_gpu, d_ref)
+    assert rel_l2(s_gpu, s_ref) < TOL, rel_l2(s_gpu, s_ref)
