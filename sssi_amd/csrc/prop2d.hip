@@ -137,12 +137,16 @@ class Prop2D final : public Prop {
     // handle's stream -- keep the fastest, and cache it for the shape.
     void tune_layout() {
         const TuneKey key{2, nz, nx, 1, N, nB};
+        static const int cand[][2] = {{4, 0}, {4, 4}, {4, 8}, {8, 0}, {8, 4}};
         int a = 0, b = 0;
         if (tune_lookup(key, a, b)) {
-            g->set_layout(a, b);
-            return;
+            // a cache entry is used only if it names an instantiated layout
+            for (const auto &c : cand)
+                if (c[0] == a && c[1] == b) {
+                    g->set_layout(a, b);
+                    return;
+                }
         }
-        static const int cand[][2] = {{4, 0}, {4, 4}, {4, 8}, {8, 0}, {8, 4}};
         constexpr int kWarm = 4, kTuneSteps = 40;
         hipEvent_t e0, e1;
         SSSI_HIP(hipEventCreate(&e0));

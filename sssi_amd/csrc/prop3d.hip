@@ -783,7 +783,8 @@ class Prop3D final : public Prop {
     void tune_layout() {
         const TuneKey key{3, nz, nx, ny, N, nB};
         int a = 0, b = 0;
-        if (tune_lookup(key, a, b)) {
+        // a cache entry is used only if it is a valid layout (as for SSSI_TY3 / SSSI_ZC3)
+        if (tune_lookup(key, a, b) && (a == 4 || a == 8 || a == 16) && b >= 8 && b <= 4096) {
             ty = a;
             zc = b;
             return;

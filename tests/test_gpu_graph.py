@@ -125,3 +125,34 @@ def test_autotuned_layouts_match_oracle_and_are_cached():
     d_ref, _ = oracle.fwd_cpml_2d(v, nt, 4, 20, 5.0, 5.0, dt, src_iz=[0], src_ix=[200],
                                   wavelet=w)
     assert np.linalg.norm(d - d_ref) / np.linalg.norm(d_ref) < 1e-5
+
+
+def test_invalid_tune_cache_entries_are_ignored(tmp_path):
+    """Entries of SSSI_TUNE_CACHE that name no valid layout (a hand-edited or
+    damaged file) are ignored and the shape is tuned afresh.  The file is read
+    once per process, so the handles are made in a child process."""
+    import json
+    import subprocess
+    import sys
+    cache = tmp_path / "tune.txt"
+    # "dim nz nx ny N nB a b" for the two shapes below, with layouts no kernel has
+    cache.write_text("2 64 96 1 3 8 7 3\n3 24 40 36 2 6 5 0\n")
+    code = (
+        "import json, numpy as np, sssi_amd\n"
+        "from sssi_amd import synth\n"
+        "sssi_amd.set_autotune(True)\n"
+        "v2 = synth.layered_2d(64, 96, 3, 1500.0, 2500.0, seed=1)\n"
+        "p2 = sssi_amd.Propagator(v2, 3, 8, 10.0, 10.0, 1e-3)\n"
+        "v3 = synth.cake_3d_dipping(24, 40, 36, 3, 1500.0, 2500.0, seed=2)\n"
+        "p3 = sssi_amd.Propagator(v3, 2, 6, 10.0, 10.0, 1e-3, dy=10.0)\n"
+        "p2.run(5); p3.run(5); p2.sync(); p3.sync()\n"
+        "print(json.dumps([list(p2.layout()), list(p3.layout())]))\n")
+    env = dict(os.environ, SSSI_TUNE_CACHE=str(cache))
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=240, cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert r.returncode == 0, r.stderr[-3000:]
+    (rz, ng), (ty, zc) = json.loads(r.stdout.strip().splitlines()[-1])
+    assert (rz, ng) in {(4, 0), (4, 4), (4, 8), (8, 0), (8, 4)}
+    assert ty in (4, 8, 16) and 8 <= zc <= 4096
+    # the re-tuned layouts were appended to the cache
+    assert len(cache.read_text().splitlines()) == 4
