@@ -117,16 +117,21 @@ struct Tile3 {
     static constexpr int NT = 64 * TY;
     static constexpr int PX = kTX + 2 * L;  // LDS plane tile: x0-L .. x0+63+L
     static constexpr int PY = TY + 2 * L;   //                 y0-L .. y0+TY-1+L
-    static constexpr int HN = PX * PY - kTX * TY;  // halo values per plane
+    // halo values per plane: the y halo at the tile's own 64 columns and the x
+    // halo at its own rows.  The corners (y halo x x halo) are never read: the
+    // y stencil runs down owned columns, the x stencil along owned rows, in
+    // both sweeps.  The y-halo rows are then 64 aligned floats, two cache lines.
+    static constexpr int HN = 2 * L * kTX + TY * 2 * L;
     static constexpr int HPT = (HN + NT - 1) / NT;
     static constexpr int AXW = kTX + L;  // A_x half points x0-N .. x0+63+N-1 per row
     static constexpr int AYH = TY + L;   // A_y half points y0-N .. y0+TY-1+N-1 per column
     static constexpr int QY = (AYH + TY - 1) / TY;  // A_y half points per thread
 };
 
-// Halo slots of this thread in the LDS plane tile (fixed for the sweep).
-// Threads without a slot load a valid cell that is never written to LDS (an
-// unconditional load keeps the prefetched registers out of branches).
+// Halo slots of this thread in the LDS plane tile (fixed for the sweep), the
+// cells of Tile3::HN.  Threads without a slot load a valid cell that is never
+// written to LDS (an unconditional load keeps the prefetched registers out of
+// branches).
 template <int N, int TY>
 __device__ __forceinline__ void halo_slots3(const Cpml3dArgs &a, int tid, int x0, int y0,
                                             int (&h_lds)[Tile3<N, TY>::HPT],
@@ -138,19 +143,20 @@ __device__ __forceinline__ void halo_slots3(const Cpml3dArgs &a, int tid, int x0
         const int e = tid + T::NT * u;
         int lidx = -1, gy = 0, gx = 0;
         if (e < T::HN) {
-            // enumerate tile cells that are not owned: rows of the y halo first
-            const int top = L * PX;      // y0-L .. y0-1
-            const int mid = TY * 2 * L;  // x halo of the owned rows
+            // the y halo above (rows y0-L .. y0-1, owned columns), the x halo
+            // of the owned rows, the y halo below (rows y0+TY .. y0+TY+L-1)
+            const int top = L * kTX;
+            const int mid = TY * 2 * L;
             if (e < top) {
-                const int r = e / PX, c = e - r * PX;
-                lidx = r * PX + c; gy = y0 - L + r; gx = x0 - L + c;
+                const int r = e / kTX, c = e - r * kTX;
+                lidx = r * PX + L + c; gy = y0 - L + r; gx = x0 + c;
             } else if (e < top + mid) {
                 const int q = e - top, r = q / (2 * L), c = q - r * (2 * L);
                 const int cc = c < L ? c : kTX + c;
                 lidx = (L + r) * PX + cc; gy = y0 + r; gx = x0 - L + cc;
             } else {
-                const int q = e - top - mid, r = q / PX, c = q - r * PX;
-                lidx = (L + TY + r) * PX + c; gy = y0 + TY + r; gx = x0 - L + c;
+                const int q = e - top - mid, r = q / kTX, c = q - r * kTX;
+                lidx = (L + TY + r) * PX + L + c; gy = y0 + TY + r; gx = x0 + c;
             }
         }
         h_lds[u] = lidx;
