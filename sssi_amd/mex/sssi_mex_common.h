@@ -90,7 +90,14 @@ static void sssi_mex_read_points(const mxArray *s, int want_iy, sssi_mex_points 
                               names[k]);
         dst[k] = (int *)mxCalloc(p->n, sizeof(int));
         const double *v = mxGetPr(f);
-        for (int q = 0; q < p->n; ++q) dst[k][q] = (int)v[q];
+        for (int q = 0; q < p->n; ++q) {
+            /* whole, non-negative and finite (NaN fails the comparison), so the
+             * cast is exact; the grid bounds are checked by the library */
+            if (!(v[q] >= 0.0 && v[q] <= 2147483647.0) || v[q] != (double)(int)v[q])
+                mexErrMsgIdAndTxt("sssi:source", "source.%s entries must be non-negative integers",
+                                  names[k]);
+            dst[k][q] = (int)v[q];
+        }
     }
     p->iz = dst[0];
     p->ix = dst[1];

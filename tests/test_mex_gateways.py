@@ -96,6 +96,12 @@ def test_struct_source_validation():
         call("fwdTimeCpmlFor3dAw_mex", v3, {"iz": [0.0], "ix": [3.0], "wavelet": w},
              2, 2, 10.0, 10.0, 10.0, 1e-3)
     assert e.value.ident == "sssi:source" and "iy" in e.value.msg
+    # indices must be whole, non-negative and finite before they reach the library
+    for bad in (1.5, -1.0, float("nan"), float("inf")):
+        with pytest.raises(MexError) as e:
+            call("fwdTimeCpmlFor2dAw_mex", v, {"iz": [0.0], "ix": [bad], "wavelet": w[:, None]},
+                 2, 10, 10.0, 10.0, 1e-3)
+        assert e.value.ident == "sssi:source" and "integers" in e.value.msg
 
 
 def test_rvs_validation():
