@@ -82,7 +82,7 @@ void run_oneshot(Prop &P, int64_t nt, bool reverse, double *data, double *snap, 
         const int64_t per = std::max<int64_t>(1, (int64_t)(256ll << 20) / (cells * 4));
         const int64_t K = std::min(per, nsnap);
         DevBuf<float> stage((size_t)K * cells);
-        std::vector<float> host((size_t)K * cells);
+        HostBuf<float> host((size_t)K * cells);
         int64_t done = 0;  // steps executed
         while (done < nt) {
             // slots covered by this chunk
@@ -105,20 +105,24 @@ void run_oneshot(Prop &P, int64_t nt, bool reverse, double *data, double *snap, 
             sink.slot0 = s_lo;
             P.run(t0, count, reverse, data != nullptr, &sink, nullptr);
             const int64_t ns = s_hi - s_lo + 1;
-            SSSI_HIP(hipMemcpyAsync(host.data(), stage.p, (size_t)ns * cells * sizeof(float),
+            SSSI_HIP(hipMemcpyAsync(host.p, stage.p, (size_t)ns * cells * sizeof(float),
                                     hipMemcpyDeviceToHost, P.stream));
             SSSI_HIP(hipStreamSynchronize(P.stream));
             // device slice is x fastest, [z][y][x]: (i*ny + k)*nx + j;
-            // reference layout is z fastest: (k*nx + j)*nz + i
-            const int64_t nz = P.nz, nx = P.nx, ny = P.ny;
-            for (int64_t s = 0; s < ns; ++s) {
-                const float *src = host.data() + s * cells;
-                double *dst = snap + (s_lo + s) * cells;
-                for (int64_t i = 0; i < nz; ++i)
-                    for (int64_t k = 0; k < ny; ++k)
-                        for (int64_t j = 0; j < nx; ++j)
+            // reference layout is z fastest: (k*nx + j)*nz + i.  Split over
+            // host threads by (slice, y, 64-column block); each item writes
+            // whole z columns.
+            const int64_t nz = P.nz, nx = P.nx, ny = P.ny, nxb = (nx + 63) / 64;
+            parallel_ranges(ns * ny * nxb, [&](int64_t a, int64_t b) {
+                for (int64_t it = a; it < b; ++it) {
+                    const int64_t s = it / (ny * nxb), k = (it / nxb) % ny, jb = it % nxb;
+                    const float *src = host.p + s * cells;
+                    double *dst = snap + (s_lo + s) * cells;
+                    for (int64_t j = jb * 64; j < std::min(nx, jb * 64 + 64); ++j)
+                        for (int64_t i = 0; i < nz; ++i)
                             dst[(k * nx + j) * nz + i] = src[(i * ny + k) * nx + j];
-            }
+                }
+            }, 1);
             done += count;
         }
     }

@@ -9,8 +9,10 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sssi.h"
@@ -71,6 +73,47 @@ struct DevBuf {
     }
     size_t bytes() const { return n * sizeof(T); }
 };
+
+// Owning pinned host buffer (hipHostMalloc): copies between it and the device
+// run at link speed rather than through the runtime's pageable staging.
+template <class T>
+struct HostBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    HostBuf() = default;
+    explicit HostBuf(size_t count) { alloc(count); }
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) SSSI_HIP(hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault));
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// f(begin, end) over a split of [0, n) on up to 16 host threads; one thread
+// when n / min_chunk < 2.  f must not throw.
+template <class F>
+void parallel_ranges(int64_t n, F f, int64_t min_chunk = 4096) {
+    const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({16, hw, n / std::max<int64_t>(1, min_chunk)}));
+    if (nt <= 1) {
+        if (n > 0) f((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (int64_t k = 1; k < nt; ++k) th.emplace_back(f, n * k / nt, n * (k + 1) / nt);
+    f((int64_t)0, n / nt);
+    for (auto &t : th) t.join();
+}
 
 // Staggered coefficients c_j (dCoef(N,'s'), finiteDifference.c:57-115): exact
 // rational c_j = (1/(2j-1)) prod_{m!=j} (2m-1)^2/((2m-1)^2-(2j-1)^2), reduced

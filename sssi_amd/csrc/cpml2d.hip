@@ -660,13 +660,24 @@ Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB
     p[0].alloc(np);
     p[1].alloc(np);
     vdt2.alloc(np);
-    std::vector<float> hv(np, 0.f);
-    for (int64_t j = 0; j < nx; ++j)
-        for (int64_t i = 0; i < nz; ++i) {
-            const double vv = v[j * nz + i];
-            hv[(size_t)(i + kHZ) * pitch + j + kHX] = (float)((vv * dt) * (vv * dt));
-        }
-    SSSI_HIP(hipMemcpy(vdt2.p, hv.data(), np * sizeof(float), hipMemcpyHostToDevice));
+    {
+        // v (column-major, z fastest) -> padded x-fastest rows, zero halo.  The
+        // zero fill and the transpose are split over host threads; the
+        // transpose reads each column contiguously and writes one word into
+        // each of nz rows, which stay in cache from one column to the next.
+        HostBuf<float> hv(np);
+        parallel_ranges((int64_t)np, [&](int64_t a, int64_t b) {
+            std::fill(hv.p + a, hv.p + b, 0.f);
+        }, 1 << 18);
+        parallel_ranges((int64_t)nx, [&](int64_t j0, int64_t j1) {
+            for (int64_t j = j0; j < j1; ++j)
+                for (int64_t i = 0; i < nz; ++i) {
+                    const double vv = v[j * nz + i];
+                    hv.p[(size_t)(i + kHZ) * pitch + j + kHX] = (float)((vv * dt) * (vv * dt));
+                }
+        }, 16);
+        SSSI_HIP(hipMemcpy(vdt2.p, hv.p, np * sizeof(float), hipMemcpyHostToDevice));
+    }
     double c[8];
     staggered_coefficients(N, c);
     for (int k = 0; k < N; ++k) {

@@ -343,11 +343,12 @@ class Prop2D final : public Prop {
     void get_data(double *data, int64_t nt) override {
         if (nt > rec_nt || !rec.p)
             fail(SSSI_ESTATE, "get_data: no recorded traces for %lld steps", (long long)nt);
-        std::vector<float> h((size_t)nt * nx);
-        SSSI_HIP(hipMemcpyAsync(h.data(), rec.p, h.size() * sizeof(float), hipMemcpyDeviceToHost,
-                                stream));
+        HostBuf<float> h((size_t)nt * nx);
+        SSSI_HIP(hipMemcpyAsync(h.p, rec.p, h.bytes(), hipMemcpyDeviceToHost, stream));
         SSSI_HIP(hipStreamSynchronize(stream));
-        for (size_t q = 0; q < h.size(); ++q) data[q] = h[q];  // [t][j] == data(nx,nt)
+        parallel_ranges((int64_t)h.n, [&](int64_t a, int64_t b) {
+            for (int64_t q = a; q < b; ++q) data[q] = h.p[q];  // [t][j] == data(nx,nt)
+        }, 1 << 16);
     }
 
     // sponge levels: interior of sp[which], optionally times the second damping factor

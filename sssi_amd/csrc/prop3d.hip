@@ -688,15 +688,24 @@ class Prop3D final : public Prop {
         p[1].alloc(np);
         vfield.alloc(np);
         {
-            std::vector<float> hv(np, 0.f);
-            for (int64_t k = 0; k < ny; ++k)
-                for (int64_t j = 0; j < nx; ++j)
-                    for (int64_t i = 0; i < nz; ++i) {
-                        const double vv = v[(k * nx + j) * nz + i];
-                        hv[((size_t)(i + kHZ) * npy + k + kHY) * pitch + j + kHX] =
-                            (float)((vv * dt) * (vv * dt));
-                    }
-            SSSI_HIP(hipMemcpy(vfield.p, hv.data(), np * sizeof(float), hipMemcpyHostToDevice));
+            // v (z fastest) -> padded [z][y][x], zero halo.  The zero fill and
+            // the transpose are split over host threads; the transpose reads
+            // each (j, k) column contiguously and writes one word into each of
+            // nz plane rows, which stay in cache from one column to the next.
+            HostBuf<float> hv(np);
+            parallel_ranges((int64_t)np, [&](int64_t a, int64_t b) {
+                std::fill(hv.p + a, hv.p + b, 0.f);
+            }, 1 << 18);
+            parallel_ranges(ny, [&](int64_t k0, int64_t k1) {
+                for (int64_t k = k0; k < k1; ++k)
+                    for (int64_t j = 0; j < nx; ++j)
+                        for (int64_t i = 0; i < nz; ++i) {
+                            const double vv = v[(k * nx + j) * nz + i];
+                            hv.p[((size_t)(i + kHZ) * npy + k + kHY) * pitch + j + kHX] =
+                                (float)((vv * dt) * (vv * dt));
+                        }
+            }, 1);
+            SSSI_HIP(hipMemcpy(vfield.p, hv.p, np * sizeof(float), hipMemcpyHostToDevice));
         }
         double c[8];
         staggered_coefficients(N, c);
@@ -1003,11 +1012,12 @@ class Prop3D final : public Prop {
 
     void get_data(double *data, int64_t nt) override {
         if (nt > rec_nt || !rec.p) fail(SSSI_ESTATE, "get_data: no recorded traces");
-        std::vector<float> h((size_t)nt * nx * ny);
-        SSSI_HIP(hipMemcpyAsync(h.data(), rec.p, h.size() * sizeof(float), hipMemcpyDeviceToHost,
-                                stream));
+        HostBuf<float> h((size_t)nt * nx * ny);
+        SSSI_HIP(hipMemcpyAsync(h.p, rec.p, h.bytes(), hipMemcpyDeviceToHost, stream));
         SSSI_HIP(hipStreamSynchronize(stream));
-        for (size_t q = 0; q < h.size(); ++q) data[q] = h[q];  // [t][k][j] == data(nx,ny,nt)
+        parallel_ranges((int64_t)h.n, [&](int64_t a, int64_t b) {
+            for (int64_t q = a; q < b; ++q) data[q] = h.p[q];  // [t][k][j] == data(nx,ny,nt)
+        }, 1 << 16);
     }
 
     void get_field(double *field) override {
