@@ -77,7 +77,8 @@ class Cpml2dGrid {
     // rz 0 / ng -1: the SSSI_RZ / SSSI_NG environment variables or the defaults
     Cpml2dGrid(const double *v, int64_t nz, int64_t nx, int N, int nB, double dz, double dx,
                double dt, int rz = 0, int ng = -1);
-    // tile height (4 or 8) and interior march groups (0, 4, 8 with rz 4): the
+    // tile height (4 or 8) and interior march groups (0, 2, 4; 3 and 8 with
+    // rz 4): the
     // block geometry only; arrays and results at one rz do not depend on ng
     void set_layout(int rz, int ng);
     int ntx() const { return (int)((nx + kTX - 1) / kTX); }
@@ -108,14 +109,16 @@ struct DevSources {
 };
 
 // Interior march groups (rz rows each) per wave for a tile height rz, from
-// the environment variable SSSI_NG (default 0: tile path only).
+// the environment variable SSSI_NG (default 3 at rz 4, 0 -- tile path only --
+// at rz 8).
 int march_groups(int rz);
 
 // Block launch order for a set of jobs: (job, block) pairs; block >= 0 is a
 // 4-tile block of the tile path, block < 0 the interior march block -1-block.
-// Interior march blocks of every job come first, then the strip tiles, then
-// the remaining interior tiles; the march blocks and interior tiles are each
-// split into 8 contiguous bands (block b is dispatched to XCD b % 8).
+// Strip tiles of every job come first (they run longest), then the interior
+// march blocks, then the remaining interior tiles; the march blocks and
+// interior tiles are each split into 8 contiguous bands (block b is
+// dispatched to XCD b % 8).
 void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector<int2> &order);
 
 // Launch one step of a single job (descriptor by value).  With `base`, the

@@ -733,7 +733,7 @@ bool Cpml2dGrid::block_is_strip(int tile) const {
 
 void Cpml2dGrid::set_layout(int rz_, int ng_) {
     if (!(rz_ == 4 || rz_ == 8)) fail(SSSI_EINVAL, "tile height %d (4 or 8)", rz_);
-    if (!(ng_ == 0 || ng_ == 4 || (ng_ == 8 && rz_ == 4)))
+    if (!(ng_ == 0 || ng_ == 2 || ng_ == 4 || ((ng_ == 3 || ng_ == 8) && rz_ == 4)))
         fail(SSSI_EINVAL, "march groups %d with tile height %d", ng_, rz_);
     rz = rz_;
     ng = ng_;
@@ -774,9 +774,10 @@ static int env_int(const char *name, int dflt) {
 }
 
 int march_groups(int rz) {
-    int ng = env_int("SSSI_NG", 0);
+    int ng = env_int("SSSI_NG", rz == 4 ? 3 : 0);
     if (rz == 8 && ng > 4) ng = 4;
-    return (ng == 4 || ng == 8) ? ng : 0;
+    if (rz == 8 && ng == 3) ng = 2;
+    return (ng >= 2 && ng <= 4) || ng == 8 ? ng : 0;
 }
 
 void Cpml2dGrid::zero_state(hipStream_t s) {
@@ -891,17 +892,20 @@ void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector
         }
     }
     order.clear();
-    append_banded(order, march);
     order.insert(order.end(), strip.begin(), strip.end());
+    append_banded(order, march);
     append_banded(order, inner);
 }
 
 // Instantiated layouts: (rz, march groups); groups 0 is the tile path alone.
 #define SSSI_NRZ_CASES(NN, CALL)                                                          \
     case NN * 100 + 4 * 10 + 0: CALL(NN, 4, 0); break;                                    \
+    case NN * 100 + 4 * 10 + 2: CALL(NN, 4, 2); break;                                    \
+    case NN * 100 + 4 * 10 + 3: CALL(NN, 4, 3); break;                                    \
     case NN * 100 + 4 * 10 + 4: CALL(NN, 4, 4); break;                                    \
     case NN * 100 + 4 * 10 + 8: CALL(NN, 4, 8); break;                                    \
     case NN * 100 + 8 * 10 + 0: CALL(NN, 8, 0); break;                                    \
+    case NN * 100 + 8 * 10 + 2: CALL(NN, 8, 2); break;                                    \
     case NN * 100 + 8 * 10 + 4: CALL(NN, 8, 4); break;
 #define SSSI_NRZ_SWITCH(N, RZ, NG, CALL)                                                   \
     switch ((N) * 100 + (RZ) * 10 + (NG)) {                                               \

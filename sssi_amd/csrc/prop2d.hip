@@ -145,7 +145,8 @@ class Prop2D final : public Prop {
     // handle's stream -- keep the fastest, and cache it for the shape.
     void tune_layout() {
         const TuneKey key{2, nz, nx, 1, N, nB};
-        static const int cand[][2] = {{4, 0}, {4, 4}, {4, 8}, {8, 0}, {8, 4}};
+        static const int cand[][2] = {{4, 0}, {4, 2}, {4, 3}, {4, 4}, {4, 8},
+                                      {8, 0}, {8, 2}, {8, 4}};
         int a = 0, b = 0;
         if (tune_lookup(key, a, b)) {
             // a cache entry is used only if it names an instantiated layout

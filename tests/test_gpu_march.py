@@ -7,8 +7,8 @@ bits; each is checked against the FP64 oracle elsewhere) -- forward
 seismograms and snapshots with sources inside and outside the march region,
 reverse propagation with surface injection, a model with no damping strips
 (the march then starts at column 0), and batched BF16-checkpoint RTM images.
-The march layout is chosen per grid from SSSI_NG (0 = tile path only) and
-SSSI_RZ when the grid is created.
+The march layout is chosen per grid from SSSI_NG (0 = tile path only;
+default 3 at rz 4) and SSSI_RZ when the grid is created.
 """
 import os
 
@@ -42,7 +42,7 @@ def model(nz, nx, seed):
     return np.clip(v, 1500.0, 4000.0)
 
 
-LAYOUTS = [(4, 4), (8, 4), (4, 8)]
+LAYOUTS = [(2, 4), (3, 4), (4, 4), (8, 4), (2, 8), (4, 8)]
 
 
 @pytest.mark.parametrize("N", [2, 5, 8])
