@@ -43,6 +43,11 @@ __device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32
                                        uint32_t soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
 }
+// Byte offset marking a lane with no value in a buffer of fewer than 2^31
+// bytes: the range check covers the vector offset, so a load there returns 0
+// and a store is dropped.  Offsets of the form kOOB + k (k < 2^31) stay out
+// of range.
+constexpr uint32_t kOOB = 0x80000000u;
 
 // Load of wave-uniform data that no launch of this kernel writes (source
 // tables, checkpoint scales of earlier steps): through the constant address
@@ -334,8 +339,18 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
         // composite stencil otherwise (equal there, where Phi = Psi = 0).
         // ---- z: two-stage CPML form where a z strip is within reach
         if (zstrip) {
-            const float *zphi_in = J.zphi[par];
-            float *zphi_out = J.zphi[par ^ 1];
+            const uint32_t zbytes = (uint32_t)nB * (uint32_t)nx * 4u;
+            const __amdgpu_buffer_rsrc_t rZi = buf_rsrc(J.zphi[par], zbytes);
+            const __amdgpu_buffer_rsrc_t rZo = buf_rsrc(J.zphi[par ^ 1], zbytes);
+            const __amdgpu_buffer_rsrc_t rZb = buf_rsrc(J.zbm1, zbytes);
+            const __amdgpu_buffer_rsrc_t rZs = buf_rsrc(J.zpsi, zbytes);
+            // byte offset of this lane's value in the strip row of depth h
+            // (wave-uniform test), kOOB where there is none
+            const uint32_t zcol = jin ? (uint32_t)j * 4u : kOOB;
+            const uint32_t nxb = (uint32_t)nx * 4u;
+            auto zoff = [&](int h) -> uint32_t {
+                return h >= zb0 && h < nz ? zcol + (uint32_t)(h - zb0) * nxb : kOOB;
+            };
             // A_z at half points h = zs-N+q (q < RZ+L): D+p(h) = sum c_k (p(h+1+k) - p(h-k))
             float Az[RZ + L];
 #pragma unroll
@@ -345,54 +360,39 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
                 for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], col[q + N + k] - col[q + N - 1 - k], d);
                 Az[q] = d;
             }
-            if (zstrip && jin) {
-                // all z-strip memory variables of the tile in one batch of loads;
-                // (b-1) at half point h and at row i = h share one index
-                float phz[RZ + L], bmh[RZ + L], psz[RZ];
+            // all z-strip memory variables of the tile in one batch of loads
+            // (zero for lanes outside the model); (b-1) at half point h and at
+            // row i = h share one index
+            float phz[RZ + L], bmh[RZ + L], psz[RZ];
 #pragma unroll
-                for (int q = 0; q < RZ + L; ++q) {
-                    const int h = zs - N + q;
-                    const bool in = h >= zb0 && h < nz;
-                    const size_t o = in ? (size_t)(h - zb0) * nx + j : 0;
-                    phz[q] = in ? zphi_in[o] : 0.f;
-                    bmh[q] = in ? J.zbm1[o] : 0.f;
+            for (int q = 0; q < RZ + L; ++q) {
+                const uint32_t o = zoff(zs - N + q);
+                phz[q] = buf_ld(rZi, o, 0);
+                bmh[q] = buf_ld(rZb, o, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) psz[r] = buf_ld(rZs, zoff(zs + r), 0);
+#pragma unroll
+            for (int q = 0; q < RZ + L; ++q) {
+                const int h = zs - N + q;
+                if (h >= zb0 && h < nz) {
+                    const float phi = fmaf(bmh[q], phz[q] + Az[q], phz[q]);  // b*phi + (b-1)*D
+                    Az[q] += phi;
+                    if (h >= zs && h < zs + RZ) buf_st(phi, rZo, zoff(h), 0);
                 }
+            }
 #pragma unroll
-                for (int r = 0; r < RZ; ++r) {
-                    const int i = zs + r;
-                    const bool in = i >= zb0 && i < nz;
-                    psz[r] = in ? J.zpsi[(size_t)(i - zb0) * nx + j] : 0.f;
+            for (int r = 0; r < RZ; ++r) {
+                float d = 0.f;
+#pragma unroll
+                for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
+                const int i = zs + r;
+                if (i >= zb0 && i < nz) {
+                    const float psi = fmaf(bmh[r + N], psz[r] + d, psz[r]);
+                    buf_st(psi, rZs, zoff(i), 0);
+                    d += psi;
                 }
-#pragma unroll
-                for (int q = 0; q < RZ + L; ++q) {
-                    const int h = zs - N + q;
-                    if (h >= zb0 && h < nz) {
-                        const float phi = fmaf(bmh[q], phz[q] + Az[q], phz[q]);  // b*phi + (b-1)*D
-                        Az[q] += phi;
-                        if (h >= zs && h < zs + RZ) zphi_out[(size_t)(h - zb0) * nx + j] = phi;
-                    }
-                }
-#pragma unroll
-                for (int r = 0; r < RZ; ++r) {
-                    float d = 0.f;
-#pragma unroll
-                    for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
-                    const int i = zs + r;
-                    if (i >= zb0 && i < nz) {
-                        const float psi = fmaf(bmh[r + N], psz[r] + d, psz[r]);
-                        J.zpsi[(size_t)(i - zb0) * nx + j] = psi;
-                        d += psi;
-                    }
-                    lap[r] = d;
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < RZ; ++r) {
-                    float d = 0.f;
-#pragma unroll
-                    for (int k = 0; k < N; ++k) d = fmaf(J.cz[k], Az[r + N + k] - Az[r + N - 1 - k], d);
-                    lap[r] = d;
-                }
+                lap[r] = d;
             }
         } else {
 #pragma unroll
@@ -407,36 +407,50 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
         // ---- x: two-stage CPML form where an x strip is within reach
         if (xstrip) {
             // A_x at half points h = x0-N+q, q < 64+L: lane q, lanes < L also q = 64+lane
-            const float *xphi_in = J.xphi[par];
-            float *xphi_out = J.xphi[par ^ 1];
-            float *sa = lds_a + w * (RZ * AW);
             const int nb2 = 2 * nB;
-            const bool xown = xstrip && jin && (j < nB || j >= nx - nB);
-            const int xs = j < nB ? j : j - (nx - nb2);
+            const uint32_t xbytes = (uint32_t)nz * (uint32_t)nb2 * 4u;
+            const __amdgpu_buffer_rsrc_t rXi = buf_rsrc(J.xphi[par], xbytes);
+            const __amdgpu_buffer_rsrc_t rXo = buf_rsrc(J.xphi[par ^ 1], xbytes);
+            const __amdgpu_buffer_rsrc_t rXb = buf_rsrc(J.xbm1, xbytes);
+            const __amdgpu_buffer_rsrc_t rXs = buf_rsrc(J.xpsi, xbytes);
+            float *sa = lds_a + w * (RZ * AW);
+            // byte offset within a strip row of column h, kOOB if h lies in no
+            // x strip; a row adds i * nb2 * 4 (< 2^31), which keeps kOOB out of range
+            auto xcol = [&](int h) -> uint32_t {
+                return h >= 0 && h < nx && (h < nB || h >= nx - nB)
+                           ? (uint32_t)(h < nB ? h : h - (nx - nb2)) * 4u
+                           : kOOB;
+            };
+            uint32_t xc[2];
+            xc[0] = xcol(x0 - N + lane);
+            xc[1] = lane < L ? xcol(x0 - N + kTX + lane) : kOOB;
+            // half points this lane owns (Phi written by this tile): x0 .. x0+63
+            const bool hown[2] = {lane >= N, lane < N};
+            const bool xown = jin && (j < nB || j >= nx - nB);
+            const uint32_t xo = xown ? xcol(j) : kOOB;
+            const uint32_t rb2 = (uint32_t)nb2 * 4u;
             // all x-strip memory variables of the tile in one batch of loads
             float phx[2][RZ], bmx[2][RZ], psx[RZ], bmo[RZ];
 #pragma unroll
             for (int r = 0; r < RZ; ++r) {
                 const int i = zs + r;
+                const bool rin = i < nz;
+                const uint32_t ro = (uint32_t)i * rb2;
 #pragma unroll
                 for (int half = 0; half < 2; ++half) {
-                    const int q = lane + half * kTX;
-                    const int h = x0 - N + q;
-                    const bool in = xstrip && (half == 0 || lane < L) && i < nz && h >= 0 && h < nx &&
-                                    (h < nB || h >= nx - nB);
-                    const size_t o = in ? (size_t)i * nb2 + (h < nB ? h : h - (nx - nb2)) : 0;
-                    phx[half][r] = in ? xphi_in[o] : 0.f;
-                    bmx[half][r] = in ? J.xbm1[o] : 0.f;
+                    const uint32_t o = rin ? xc[half] + ro : kOOB;
+                    phx[half][r] = buf_ld(rXi, o, 0);
+                    bmx[half][r] = buf_ld(rXb, o, 0);
                 }
-                const bool own = xown && i < nz;
-                const size_t o = own ? (size_t)i * nb2 + xs : 0;
-                psx[r] = own ? J.xpsi[o] : 0.f;
-                bmo[r] = own ? J.xbm1[o] : 0.f;
+                const uint32_t o = rin ? xo + ro : kOOB;
+                psx[r] = buf_ld(rXs, o, 0);
+                bmo[r] = buf_ld(rXb, o, 0);
             }
 #pragma unroll
             for (int r = 0; r < RZ; ++r) {
                 const float *row = sp + r * PW;
                 const int i = zs + r;
+                const uint32_t ro = (uint32_t)i * rb2;
 #pragma unroll
                 for (int half = 0; half < 2; ++half) {
                     const int q = lane + half * kTX;
@@ -445,12 +459,10 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
 #pragma unroll
                     for (int k = 0; k < N; ++k)
                         d = fmaf(J.cx[k], row[q + N + k] - row[q + N - 1 - k], d);
-                    const int h = x0 - N + q;
-                    if (xstrip && i < nz && h >= 0 && h < nx && (h < nB || h >= nx - nB)) {
+                    if (i < nz && xc[half] != kOOB) {
                         const float phi = fmaf(bmx[half][r], phx[half][r] + d, phx[half][r]);
                         d += phi;
-                        if (h >= x0 && h < x0 + kTX)
-                            xphi_out[(size_t)i * nb2 + (h < nB ? h : h - (nx - nb2))] = phi;
+                        if (hown[half]) buf_st(phi, rXo, xc[half] + ro, 0);
                     }
                     sa[r * AW + q] = d;
                 }
@@ -465,7 +477,7 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
                 const int i = zs + r;
                 if (xown && i < nz) {
                     const float psi = fmaf(bmo[r], psx[r] + d, psx[r]);
-                    J.xpsi[(size_t)i * nb2 + xs] = psi;
+                    buf_st(psi, rXs, xo + (uint32_t)i * rb2, 0);
                     d += psi;
                 }
                 lap[r] += d;
@@ -707,6 +719,10 @@ Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB
     w0x = (float)(-sidex);
     w0xlo = (float)(-sidex - (double)w0x);
     if (nB > 0) {
+        // strip state is addressed with 32-bit buffer offsets below 2^31 (the
+        // kernel marks lanes without a value with an offset of 2^31)
+        if ((int64_t)nB * nx * 4 >= ((int64_t)1 << 31) || nz * 2 * nB * 4 >= ((int64_t)1 << 31))
+            fail(SSSI_EINVAL, "2-D damping-strip state exceeds 2 GiB per array");
         std::vector<float> zb, xb;
         cpml2d_bm1(v, nz, nx, nB, dz, dx, dt, zb, xb);
         zbm1.alloc(zb.size());
