@@ -658,6 +658,10 @@ static void composite_weights(int N, const double *c, double *w) {
 Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB_, double dz_,
                        double dx_, double dt_, int rz_, int ng_)
     : nz(nz_), nx(nx_), N(N_), nB(nB_), dz(dz_), dx(dx_), dt(dt_) {
+    // strip state is addressed with 32-bit buffer offsets below 2^31 (the
+    // kernel marks lanes without a value with an offset of 2^31)
+    if ((int64_t)nB * nx * 4 >= ((int64_t)1 << 31) || nz * 2 * nB * 4 >= ((int64_t)1 << 31))
+        fail(SSSI_EINVAL, "2-D damping-strip state exceeds 2 GiB per array");
     int r = 4;
     if (rz_ == 4 || rz_ == 8) r = rz_;
     else if (const char *e = std::getenv("SSSI_RZ")) {
@@ -719,10 +723,6 @@ Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB
     w0x = (float)(-sidex);
     w0xlo = (float)(-sidex - (double)w0x);
     if (nB > 0) {
-        // strip state is addressed with 32-bit buffer offsets below 2^31 (the
-        // kernel marks lanes without a value with an offset of 2^31)
-        if ((int64_t)nB * nx * 4 >= ((int64_t)1 << 31) || nz * 2 * nB * 4 >= ((int64_t)1 << 31))
-            fail(SSSI_EINVAL, "2-D damping-strip state exceeds 2 GiB per array");
         std::vector<float> zb, xb;
         cpml2d_bm1(v, nz, nx, nB, dz, dx, dt, zb, xb);
         zbm1.alloc(zb.size());
