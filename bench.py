@@ -130,6 +130,47 @@ def bench_3d(args):
     }
 
 
+def bench_sponge(args):
+    """2-D Cerjan-sponge forward on the headline 2-D model and shot (1040 x 3080 ext grid,
+    nBoundary = 40, nt = 4000)."""
+    import sssi_amd
+    wl = workload()
+    V = wl["V"]
+    nt = wl["nt"] if args.nt is None else args.nt
+    P = sssi_amd.Propagator(V, 1, wl["nB"], wl["dz"], wl["dx"], wl["dt"], abc="sponge")
+    P.set_points(wl["src_iz"], wl["src_ix"], wl["wavelet"])
+    bytes_step, cells = P.alg_bytes_per_step()
+    for _ in range(args.warmup):
+        P.reset()
+        P.run(nt)
+    P.sync()
+    kern_ms, launches = 0.0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        P.reset()
+        ms, n = P.run_timed(nt)
+        kern_ms += ms
+        launches += n
+    wall = time.perf_counter() - t0
+    avg = kern_ms / 1e3 / launches
+    ach = bytes_step / avg / 1e9
+    return {
+        "metric": "Gcell-updates/s (2D acoustic sponge forward, FP32)",
+        "value": round(cells * nt * args.steps / wall / 1e9, 3), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded layered model + Gaussian perturbation, Ricker source)",
+        "config": {"workload": f"2D sponge forward, {V.shape[0]}x{V.shape[1]} ext grid, "
+                               f"nBoundary={wl['nB']}, nt={nt}, 1 shot", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("sponge2d_step_bench_traffic.json"),
+                     "kernel": "sponge2d_step", "alg_bytes_per_launch": bytes_step,
+                     "avg_launch_us": round(avg * 1e6, 3), "launches": launches},
+    }
+
+
 def workload_rtm(nshots=64, seed=5):
     from sssi_amd import synth
     nz, nx, nB, N = 600, 6000, 30, 4
@@ -297,9 +338,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true",
                     help="2d/3d: default kernel layout instead of the autotuned one")
-    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "rtm", "serve", "copy"],
-                    help="2d: the headline 2-D forward line (default); 3d / rtm: other workloads")
-    ap.add_argument("--nt", type=int, default=None, help="3d: time steps (default 2000)")
+    ap.add_argument("--workload", default="2d",
+                    choices=["2d", "3d", "sponge", "rtm", "serve", "copy"],
+                    help="2d: the headline 2-D forward line (default); the others: other "
+                         "workloads")
+    ap.add_argument("--nt", type=int, default=None,
+                    help="3d / sponge: time steps (default 2000 / 4000)")
     ap.add_argument("--shots", type=int, default=64, help="rtm: number of shots")
     ap.add_argument("--ck", default="bf16", choices=["fp16", "bf16"], help="rtm: checkpoint type")
     ap.add_argument("--jobs", type=int, default=60, help="serve: number of jobs")
@@ -312,6 +356,9 @@ def main():
         sys.exit(2)
     if args.workload == "3d":
         print(json.dumps(bench_3d(args)), flush=True)
+        return
+    if args.workload == "sponge":
+        print(json.dumps(bench_sponge(args)), flush=True)
         return
     if args.workload == "rtm":
         print(json.dumps(bench_rtm(args)), flush=True)

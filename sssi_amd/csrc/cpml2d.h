@@ -116,9 +116,9 @@ int march_groups(int rz);
 // Block launch order for a set of jobs: (job, block) pairs; block >= 0 is a
 // 4-tile block of the tile path, block < 0 the interior march block -1-block.
 // Strip tiles of every job come first (they run longest), then the interior
-// march blocks, then the remaining interior tiles; the march blocks and
-// interior tiles are each split into 8 contiguous bands (block b is
-// dispatched to XCD b % 8).
+// march blocks, then the remaining interior tiles; each of the three lists is
+// split into 8 contiguous bands (block b is dispatched to XCD b % 8), so
+// neighbouring tiles share an L2.
 void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector<int2> &order);
 
 // Launch one step of a single job (descriptor by value).  With `base`, the

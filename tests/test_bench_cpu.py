@@ -14,6 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 ALG_2D = 55_391_520     # bytes per launch, 2-D CPML 1040 x 3080, nB = 40
 ALG_3D = 1_674_886_400  # bytes per launch, 3-D CPML 420 x 440 x 440, nB = 20
+ALG_SP = 51_263_520     # bytes per launch, 2-D sponge 1040 x 3080
 
 
 def test_multi_gpu_request_exits_with_a_message():
@@ -33,6 +34,7 @@ def test_2d_workload_is_the_headline_configuration():
     # 16 B per cell, 20 B per damped-direction strip cell, 4 B per receiver
     nB = wl["nB"]
     assert 16 * nz * nx + 20 * (nB * nx + nz * 2 * nB) + 4 * nx == ALG_2D
+    assert 16 * nz * nx + 4 * nx == ALG_SP  # the sponge workload's model and shot
     assert np.array_equal(bench.workload()["V"], wl["V"])  # seeded
 
 
@@ -55,7 +57,8 @@ def test_3d_and_rtm_workloads():
 
 def test_committed_traffic_summaries():
     for name, alg in (("cpml2d_step_bench_traffic.json", ALG_2D),
-                      ("cpml3d_step_bench_traffic.json", ALG_3D)):
+                      ("cpml3d_step_bench_traffic.json", ALG_3D),
+                      ("sponge2d_step_bench_traffic.json", ALG_SP)):
         with open(os.path.join(ROOT, "profiles", name)) as f:
             d = json.load(f)
         t = bench.traffic_from_profiles(name)
