@@ -23,8 +23,8 @@ struct Job2D {
     int nz, nx, pitch, rows, ntx, nwz, nB;
     int start_step, nt, reverse;  // t = start..: forward t = rel, reverse t = nt-1-rel
     // interior march regions: rows [0, mz_rows) x column tiles [mta, mta+mntx),
-    // blocks of 4 column tiles (mnxb per row band) x (march groups * rz) rows
-    int mz_rows, mta, mntx, mnxb;
+    // blocks of one column tile x 4 wavefronts of (march groups * rz) rows
+    int mz_rows, mta, mntx;
     float cz[8], cx[8];           // c_k/dz, c_k/dx (two-stage form)
     float w0, w0lo, wz[16], wx[16];  // composite D-D+ weights / d^2 (w0 + w0lo: centre)
     float w0z, w0zlo, w0x, w0xlo;    // centre weights of the z / x stencils alone
@@ -89,8 +89,7 @@ class Cpml2dGrid {
     bool block_is_strip(int tile) const;
     // tile block covered by the interior march (not launched as a tile block)
     bool block_in_march(int tile) const;
-    int march_nxb() const { return (mntx + 3) / 4; }
-    int n_march_blocks() const { return mntx > 0 ? (mz_rows / (ng * rz)) * march_nxb() : 0; }
+    int n_march_blocks() const { return mntx > 0 ? (mz_rows / (4 * ng * rz)) * mntx : 0; }
     void zero_state(hipStream_t s);
     // descriptor with geometry, coefficients and state pointers filled in
     Job2D job() const;

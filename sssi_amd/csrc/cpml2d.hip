@@ -479,7 +479,9 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
 // one group to the next (each p row is loaded once per wave instead of
 // (RZ + 2L) / RZ times), and the loads of group g+1 are issued before group g
 // is computed, so every wave keeps memory requests in flight while it
-// computes.  Arithmetic is that of the tile path's interior branch.
+// computes.  Arithmetic is that of the tile path's interior branch.  The 4
+// wavefronts of a block march the same 64 columns, stacked in z, so the rows
+// their windows share are read by one CU.
 template <int N, int RZ, int NG>
 __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, float *lds_p) {
     constexpr int L = 2 * N - 1;
@@ -489,11 +491,11 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
     constexpr int WN = ZB + 2 * L;  // window rows z0-L .. z0+ZB-1+L
 
     const int lane = threadIdx.x & 63, w = wave_id();
-    const int mzr = mb / J.mnxb, mxb = mb - mzr * J.mnxb;
-    const int mt = 4 * mxb + w;
-    if (mt >= J.mntx) return;
+    // block mb: column tile mt of the march region, its 4 wavefronts stacked
+    // in z (wave w walks rows z0 .. z0 + ZB - 1 of the block's 4 ZB rows)
+    const int mzr = mb / J.mntx, mt = mb - mzr * J.mntx;
     const int txi = J.mta + mt;
-    const int x0 = txi * kTX, z0 = mzr * ZB;
+    const int x0 = txi * kTX, z0 = (mzr * 4 + w) * ZB;
     const StepCtx S = step_ctx(J, step);
     const uint32_t rowb = S.rowb;
     const __amdgpu_buffer_rsrc_t rP = buf_rsrc(J.p[S.par], S.nbytes);
@@ -737,7 +739,7 @@ void Cpml2dGrid::set_layout(int rz_, int ng_) {
     }
     // rows with no z strip within reach, in whole tile blocks and march bands
     const int64_t zlim = nB > 0 ? nz - nB - N + 1 : nz;
-    const int64_t unit = (int64_t)rz * std::lcm(4, ng);
+    const int64_t unit = (int64_t)4 * rz * ng;
     const int64_t rows_m = zlim >= unit ? (zlim / unit) * unit : 0;
     if (ta >= 0 && rows_m > 0) {
         mz_rows = (int)rows_m;
@@ -774,7 +776,7 @@ Job2D Cpml2dGrid::job() const {
     std::memset(&J, 0, sizeof(J));
     J.nz = (int)nz; J.nx = (int)nx; J.pitch = (int)pitch; J.rows = (int)rows;
     J.ntx = ntx(); J.nwz = nwz(); J.nB = nB;
-    J.mz_rows = mz_rows; J.mta = mta; J.mntx = mntx; J.mnxb = march_nxb();
+    J.mz_rows = mz_rows; J.mta = mta; J.mntx = mntx;
     for (int k = 0; k < 8; ++k) { J.cz[k] = cz[k]; J.cx[k] = cx[k]; }
     J.w0 = w0;
     J.w0lo = w0lo;
