@@ -94,6 +94,32 @@ def test_sponge_fwd_rvs_match_oracle():
     assert rel_l2(r_gpu, r_ref) < TOL
 
 
+@pytest.mark.parametrize("nz,nx,nB", [(5, 3, 1), (17, 65, 8), (33, 130, 0), (70, 64, 31)])
+def test_sponge_odd_shapes_match_oracle(nz, nx, nB):
+    """Grids smaller than one 64 x 16 block tile or not a multiple of it: rows
+    and columns past the model compute nothing that is stored, and the zero
+    padding they would overwrite stays the boundary.  Several sources, at the
+    corners and on the bottom row, sum per cell in source order."""
+    v = _case(nz, nx, 50 + nz)
+    nt, dt = 150, 1e-3
+    w = synth.ricker(20, nt, dt)
+    iz = [0, nz - 1, nz - 1, nz // 2, 0]
+    ix = [0, nx - 1, 0, nx // 2, 0]
+    ww = np.stack([w * (1.0 + 0.25 * k) for k in range(len(iz))], axis=1)
+    d_ref, s_ref = oracle.fwd_sponge_2d(v, nt, nB, 10.0, 10.0, dt, src_iz=iz, src_ix=ix,
+                                        wavelet=ww, want_snapshot=True, snap_stride=7)
+    d_gpu, s_gpu = sssi_amd.fwd_sponge_2d_points(v, nt, nB, 10.0, 10.0, dt, iz, ix, ww,
+                                                 want_snapshot=True, snap_stride=7)
+    assert rel_l2(d_gpu, d_ref) < TOL
+    assert rel_l2(s_gpu, s_ref) < TOL
+    m_ref, r_ref = oracle.rvs_sponge_2d(v, d_ref, nB, 10.0, 10.0, dt, want_snapshot=True,
+                                        snap_stride=5)
+    m_gpu, r_gpu = sssi_amd.rvs_sponge_2d_ex(v, d_ref, nB, 10.0, 10.0, dt, want_snapshot=True,
+                                             snap_stride=5)
+    assert rel_l2(m_gpu, m_ref) < TOL
+    assert rel_l2(r_gpu, r_ref) < TOL
+
+
 def test_handle_repeat_runs_are_identical():
     nz, nx, nt = 50, 128, 200
     v = _case(nz, nx, 41)
