@@ -573,15 +573,24 @@ __device__ __forceinline__ void cpml2d_block(const Job2D &J, int tile, int step)
 
 // step = the launch's step argument, plus a device-resident base when the
 // launch is replayed from a captured time loop (StepGraph).
+// Occupancy targets (minimum waves per SIMD) for 4-row tiles up to nDiffOrder
+// 5: 7 for the tile path alone (72 VGPRs), 6 with the interior march (76).
+// Without them the compiler scheduled the tile path at 71 VGPRs to a slower
+// order (13.42 against 12.25 us per step on the 1040 x 3080, N = 5 workload)
+// and the march at 79.  Taller tiles and higher orders need more registers
+// and keep the compiler's own choice (1: no target), as a target would spill.
 template <int N, int RZ, int NG>
-__global__ void __launch_bounds__(256) cpml2d_single(Job2D J, const int2 *__restrict__ order,
+constexpr int occ_target() { return RZ == 4 && N <= 5 ? (NG == 0 ? 7 : 6) : 1; }
+
+template <int N, int RZ, int NG>
+__global__ void __launch_bounds__(256, (occ_target<N, RZ, NG>())) cpml2d_single(Job2D J, const int2 *__restrict__ order,
                                                      int step, const int *__restrict__ base) {
     if (base) step += ld_uniform(base);
     cpml2d_block<N, RZ, NG>(J, ld_uniform(&order[blockIdx.x].y), step);
 }
 
 template <int N, int RZ, int NG>
-__global__ void __launch_bounds__(256) cpml2d_batch(const Job2D *__restrict__ jobs,
+__global__ void __launch_bounds__(256, (occ_target<N, RZ, NG>())) cpml2d_batch(const Job2D *__restrict__ jobs,
                                                     const int2 *__restrict__ order, int step,
                                                     const int *__restrict__ base) {
     if (base) step += ld_uniform(base);
