@@ -804,9 +804,11 @@ class Prop3D final : public Prop {
             zc = b;
             return;
         }
+        // z chunks: at 420 x 440 x 440, 8-row tiles, us per step at 48, 64, 80, 96,
+        // 112 planes: 457.6, 461.7, 454.2, 472.8, 472.5 (one box)
         static const int cty[] = {4, 8, 16};
-        static const int czc[] = {32, 64, 128};
-        constexpr int kWarm = 2, kTuneSteps = 6;
+        static const int czc[] = {32, 48, 64, 80, 128};
+        constexpr int kWarm = 2, kTuneSteps = 10;
         hipEvent_t e0, e1;
         SSSI_HIP(hipEventCreate(&e0));
         SSSI_HIP(hipEventCreate(&e1));

@@ -119,7 +119,7 @@ def test_autotuned_layouts_match_oracle_and_are_cached():
         v3 = synth.cake_3d_dipping(40, 70, 66, 3, 1500, 3000, seed=2)
         R = sssi_amd.Propagator(v3, 2, 6, 10.0, 10.0, 1e-3, dy=10.0)
         ty, zc = R.layout()
-        assert ty in (4, 8, 16) and zc in (32, 64, 128)
+        assert ty in (4, 8, 16) and zc in (32, 48, 64, 80, 128)
     finally:
         sssi_amd.set_autotune(False)
     d_ref, _ = oracle.fwd_cpml_2d(v, nt, 4, 20, 5.0, 5.0, dt, src_iz=[0], src_ix=[200],
