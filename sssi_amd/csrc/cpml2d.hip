@@ -511,8 +511,8 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
     float win[WN], po[ZB], v2[ZB], hal[NG][HPL];
     float *sp = lds_p + w * (RZ * PW);
     float wmax = 0.f;
-    // group q's loads: its RZ new window rows (group 0: all RZ + 2L), p_old,
-    // v^2dt^2 and x halo
+    // group q's window loads: its RZ new window rows (group 0: all RZ + 2L)
+    // and x halo, prefetched one group ahead
     auto issue = [&](int q) {
         const uint32_t sn = srow0 + q * RZ * rowb;
         if (q == 0) {
@@ -520,18 +520,30 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
             for (int r = 0; r < 2 * L; ++r) win[r] = buf_ld(rP, vcol, srow0 + (r - L) * rowb);
         }
 #pragma unroll
+        for (int r = 0; r < RZ; ++r) win[q * RZ + 2 * L + r] = buf_ld(rP, vcol, sn + (r + L) * rowb);
+#pragma unroll
+        for (int u = 0; u < HPL; ++u) hal[q][u] = buf_ld(rP, h_off[u], sn);
+    };
+    // group q's p_old and v^2dt^2, needed only by its update: issued when the
+    // group starts, ahead of the next group's prefetch (vmcnt counts in issue
+    // order, so waiting for them leaves the prefetch in flight).  Holding them
+    // one group instead of two keeps the march at 72 VGPRs (occupancy 7 at
+    // N = 5; 76 and occupancy 6 with them prefetched: 12.31 against 11.63 us
+    // per step on the 1040 x 3080 workload at NG = 3).
+    auto issue_pv = [&](int q) {
+        const uint32_t sn = srow0 + q * RZ * rowb;
+#pragma unroll
         for (int r = 0; r < RZ; ++r) {
-            win[q * RZ + 2 * L + r] = buf_ld(rP, vcol, sn + (r + L) * rowb);
             po[q * RZ + r] = buf_ld(rO, vcol, sn + r * rowb);
             v2[q * RZ + r] = buf_ld(rV, vcol, sn + r * rowb);
         }
-#pragma unroll
-        for (int u = 0; u < HPL; ++u) hal[q][u] = buf_ld(rP, h_off[u], sn);
     };
     issue(0);
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
         // keep the prefetch one group deep: nothing below moves above this point
+        __builtin_amdgcn_sched_barrier(0);
+        issue_pv(g);
         __builtin_amdgcn_sched_barrier(0);
         if (g + 1 < NG) issue(g + 1);
         const int zs = z0 + g * RZ;
@@ -574,11 +586,12 @@ __device__ __forceinline__ void cpml2d_block(const Job2D &J, int tile, int step)
 // step = the launch's step argument, plus a device-resident base when the
 // launch is replayed from a captured time loop (StepGraph).
 // Occupancy targets (minimum waves per SIMD) for 4-row tiles up to nDiffOrder
-// 5: 7 for the tile path alone (72 VGPRs), 6 with the interior march (76).
-// Without them the compiler scheduled the tile path at 71 VGPRs to a slower
-// order (13.42 against 12.25 us per step on the 1040 x 3080, N = 5 workload)
-// and the march at 79.  Taller tiles and higher orders need more registers
-// and keep the compiler's own choice (1: no target), as a target would spill.
+// 5: 7 for the tile path alone (72 VGPRs at N = 5), 6 with the interior march
+// (which compiles to 72 as well).  Without them the compiler scheduled the tile
+// path at 71 VGPRs to a slower order (13.42 against 12.25 us per step on the
+// 1040 x 3080, N = 5 workload).  Taller tiles and higher orders need more
+// registers and keep the compiler's own choice (1: no target), as a target
+// would spill.
 template <int N, int RZ, int NG>
 constexpr int occ_target() { return RZ == 4 && N <= 5 ? (NG == 0 ? 7 : 6) : 1; }
 
