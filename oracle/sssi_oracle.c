@@ -11,27 +11,37 @@
  * operations), and OpenMP splits independent columns; the per-cell result does
  * not depend on the thread count.
  *
- * Parity status (how each function is pinned; the checks live in tests/):
+ * Parity status (how each function is pinned; the checks live in
+ * tests/test_oracle.py; each one fails under a planted index mistake):
  *   orc_dcoef            - pinned: coefficient table of UserGuide.tex:339-341
  *                          (N=1..3) and the exact rational solution of the
  *                          Vandermonde system for N=1..8.
  *   orc_cpml_profiles_2d - pinned: closed form of dampPml and its invariants.
- *   orc_fwd_cpml_2d      - pinned: independent numpy restatement of
- *                          src/fd_matlab/fwdTimeCpmlFor2dAw.m, composite
- *                          stencil from exact rational weights, light cone,
- *                          linearity, mirror symmetry, convergence to the
- *                          analytic 2-D Green's function, CPML absorption.
+ *   orc_fwd_cpml_2d      - pinned: interior = composite stencil from exact
+ *                          rational weights, light cone, linearity, mirror
+ *                          symmetry, analytic 2-D Green's function, CPML
+ *                          absorption; strip conventions (b(h) at half point
+ *                          h+1/2, b(i) at i, which rows/columns are damped) by
+ *                          closed-form impulse responses in every strip and by
+ *                          the exact step and row/column at which each strip
+ *                          first changes the field.
  *   orc_rvs_cpml_2d      - pinned: equals orc_fwd_cpml_2d driven by the
- *                          time-reversed surface data (bit for bit), numpy
- *                          restatement of src/fd_matlab/rvsTimeCpmlFor2dAw.m.
+ *                          time-reversed surface data (bit for bit).
  *   orc_fwd_cpml_3d      - pinned: bit-exact equality with the 2-D path for
- *                          y-invariant problems inside the y-strip light cone,
- *                          numpy restatement of src/fwdTimeCpmlFor3dAw.m on
- *                          tiny grids, light cone.
- *   orc_*_sponge_2d      - pinned: numpy restatement of the MATLAB sponge
- *                          files, closed-form damping factors, convergence.
+ *                          y-invariant problems inside the y-strip light cone;
+ *                          closed-form impulse responses in the x, y and z
+ *                          strips with the index-unit profiles.
+ *   orc_fwd_sponge_2d    - pinned: closed-form one-, two- and three-step
+ *                          impulse responses in every region (side strips,
+ *                          bottom strip, corners, interior).
+ *   orc_rvs_sponge_2d    - pinned: equals orc_fwd_sponge_2d driven by the
+ *                          time-reversed surface data (bit for bit), model =
+ *                          damped previous level.
  *   orc_rtm_shot_cpml_2d / orc_xcorr_image - pinned: brute-force sums over
- *                          stored snapshots.
+ *                          stored snapshots and composition of the above.
+ *   Parity unpinned: the last-bit FP64 summation order of the reference
+ *   C-MEX (it cannot be built here) and MATLAB's mldivide (replaced by the
+ *   closed-form rationals).
  */
 #include "sssi_oracle.h"
 #include <math.h>
@@ -70,7 +80,7 @@ static int64_t gcd64(int64_t a, int64_t b)
     return a;
 }
 
-/* dCoef(N,'s') -- finiteDifference.c:57-115 / src/dCoef.m:34-48 solve
+/* dCoef(N,'s') -- finiteDifference.c:24-82 / src/dCoef.m:34-48 solve
  *   sum_j (2j-1)^(2i-1) c_j = delta_{i1}.
  * With a_j = (2j-1) c_j this is a Vandermonde system in x_j = (2j-1)^2 whose
  * solution is the Lagrange basis at 0:  a_j = prod_{m!=j} x_m / (x_m - x_j).
@@ -102,7 +112,7 @@ int orc_dcoef(int N, double *c)
 
 /* ------------------------------------------------------------------------ */
 /* 2-D CPML damping: fwdTimeCpmlFor2dAw_mex.c:100-145 with dampPml
- * finiteDifference.c:286-313.  d0 = -(3v)/(2L)*log(R); d = d0*(u/L)*(u/L);
+ * finiteDifference.c:253-280.  d0 = -(3v)/(2L)*log(R); d = d0*(u/L)*(u/L);
  * b = exp(-d*dt).  Left u=(nB-j)dx, right u=(j+1)dx over the last nB columns,
  * bottom u=(i+1)dz over the last nB rows; no top layer. */
 int orc_cpml_profiles_2d(const double *v, int64_t nz, int64_t nx, int nB,
@@ -257,7 +267,7 @@ static int cpml2d_init(cpml2d_t *s, const double *v, int64_t nz, int64_t nx, int
 /* One time step, fwdTimeCpmlFor2dAw_mex.c:185-270 (rvs…_mex.c:180-265).
  * s->src holds source(:,:,t) as an nz x nx slice.  Each derivative is summed
  * over k = 0..N-1 as acc = acc + c[k]*(a-b)/d, as diffOperator2d does
- * (finiteDifference.c:136-155). */
+ * (finiteDifference.c:100-147). */
 static void cpml2d_step(cpml2d_t *s)
 {
     const int64_t nz = s->nz, nx = s->nx, npz = s->npz, npx = s->npx;

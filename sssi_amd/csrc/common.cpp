@@ -43,7 +43,7 @@ void staggered_coefficients(int N, double *c) {
     }
 }
 
-// fwdTimeCpmlFor2dAw_mex.c:100-145 with dampPml (finiteDifference.c:286-313):
+// fwdTimeCpmlFor2dAw_mex.c:100-145 with dampPml (finiteDifference.c:253-280):
 // d0 = -(3 v)/(2 L) log(1e-6), d = d0 (u/L)(u/L), b = exp(-d dt).
 void cpml2d_bm1(const double *v, int64_t nz, int64_t nx, int nB, double dz, double dx, double dt,
                 std::vector<float> &zbm1, std::vector<float> &xbm1) {

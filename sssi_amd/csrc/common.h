@@ -115,7 +115,7 @@ void parallel_ranges(int64_t n, F f, int64_t min_chunk = 4096) {
     for (auto &t : th) t.join();
 }
 
-// Staggered coefficients c_j (dCoef(N,'s'), finiteDifference.c:57-115): exact
+// Staggered coefficients c_j (dCoef(N,'s'), finiteDifference.c:24-82): exact
 // rational c_j = (1/(2j-1)) prod_{m!=j} (2m-1)^2/((2m-1)^2-(2j-1)^2), reduced
 // and rounded once (numerator and denominator stay below 2^53 for N <= 8).
 void staggered_coefficients(int N, double *c);

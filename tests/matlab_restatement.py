@@ -55,7 +55,7 @@ def damp_pml(u, v, L):
 
 
 def damp_pml_cmex(u, v, L):
-    """finiteDifference.c:286-313 order: d0*(u/L)*(u/L)."""
+    """finiteDifference.c:253-280 order: d0*(u/L)*(u/L)."""
     logR = math.log(1e-6)
     d0 = -(3.0 * v) / (2 * L) * logR
     return d0 * (u / L) * (u / L)

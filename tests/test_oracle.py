@@ -3,19 +3,26 @@
 Checks, per oracle function:
   dcoef               golden coefficient table (UserGuide.tex:339-341), exact
                       Vandermonde residual for N = 1..8
-  fwd/rvs CPML 2-D    bit-exact agreement with an independent vectorised
-                      restatement of src/fd_matlab/*.m; composite stencil from
-                      exact rational weights (one and two steps from a point
-                      source, no PML); light cone; linearity; mirror symmetry;
-                      CPML absorption; analytic 2-D Green's function
+  fwd CPML 2-D        composite stencil from exact rational weights (one and
+                      two steps from a point source, no PML); light cone;
+                      linearity; mirror symmetry; CPML absorption; analytic
+                      2-D Green's function; closed-form impulse responses in
+                      every damping strip (N = 1); exact onset step and
+                      row/column of each strip (N = 1, 2, 3, 5)
   rvs CPML 2-D        equals the forward propagator driven by the time-reversed
                       surface data, bit for bit
-  3-D CPML            bit-exact vs the vectorised restatement of
-                      src/fwdTimeCpmlFor3dAw.m; equals the 2-D propagator on a
-                      y-invariant problem inside the y light cone (unit spacing,
-                      where the 3-D index-unit damping equals the 2-D one)
-  sponge fwd/rvs      bit-exact vs the restatement; closed-form damping factor
+  3-D CPML            equals the 2-D propagator on a y-invariant problem inside
+                      the y light cone (unit spacing, where the 3-D index-unit
+                      damping equals the 2-D one); closed-form impulse
+                      responses in the x, y and z strips
+  sponge fwd          closed-form one-, two- and three-step impulse responses in
+                      every region (side strips, bottom strip, corners)
+  sponge rvs          equals the forward sponge driven by the time-reversed
+                      surface data, bit for bit; model = damped previous level
   imaging             brute-force cross-correlation sums
+The vectorised restatement of the MATLAB files (matlab_restatement.py) is a
+cross-check by the same author, not a pin: every check above also fails under
+a planted index mistake in the oracle without it.
 """
 import json
 import math
@@ -423,3 +430,273 @@ def test_rtm_shot_composes_fwd_rvs_and_imaging():
     assert np.array_equal(out["data_true"], dT)
     assert np.array_equal(out["num"], num) and np.array_equal(out["den"], den)
     assert np.array_equal(out["image"], img)
+
+
+# --------------------------------------------------------------------------
+# Damping-strip conventions, pinned independently of any restatement of the
+# loops: closed-form responses to a single impulse and exact onset times.
+#
+# CPML (fwdTimeCpmlFor2dAw_mex.c:185-253): the half point h+1/2 between rows
+# h and h+1 carries Phi damped with b(h) (the row above it), and the integer
+# point i carries Psi damped with b(i).  With Phi = Psi = 0 before the step,
+# A(h+1/2) = b(h) D+p(h+1/2) and P(i) = b(i) D-A(i).  For N = 1 and an impulse
+# of value a at (i0, j0) the second step is therefore
+#   P_z(i0-1) = a b(i0-1)^2 / dz^2,   P_z(i0+1) = a b(i0+1) b(i0) / dz^2,
+#   P_z(i0)   = -a b(i0) (b(i0) + b(i0-1)) / dz^2       (x alike with b_x, dx)
+# which tells b(h) at half point h+1/2 apart from b(h+1).
+
+def _b_profiles(v, nB, dz, dx, dt):
+    """b = exp(-d dt), d = -(3 v / 2L) ln(1e-6) (u/L)^2 (src/dampPml.m:16-18) on
+    the strips of fwdTimeCpmlFor2dAw_mex.c:100-145: left u = (nB-j) dx,
+    right u = (j-(nx-nB)+1) dx, bottom u = (i-(nz-nB)+1) dz."""
+    nz, nx = v.shape
+    bx = np.ones((nz, nx))
+    bz = np.ones((nz, nx))
+    Lx, Lz = nB * dx, nB * dz
+    for j in range(nB):
+        u = (nB - j) * dx
+        bx[:, j] = np.exp(-(-(3 * v[:, j]) / (2 * Lx) * math.log(1e-6)) * (u / Lx) ** 2 * dt)
+        jr = nx - nB + j
+        u = (j + 1) * dx
+        bx[:, jr] = np.exp(-(-(3 * v[:, jr]) / (2 * Lx) * math.log(1e-6)) * (u / Lx) ** 2 * dt)
+    for i in range(nB):
+        ir = nz - nB + i
+        u = (i + 1) * dz
+        bz[ir, :] = np.exp(-(-(3 * v[ir, :]) / (2 * Lz) * math.log(1e-6)) * (u / Lz) ** 2 * dt)
+    return bx, bz
+
+
+@pytest.mark.parametrize("i0,j0", [(19, 12), (21, 3), (17, 2), (20, 25), (22, 28)])
+def test_cpml_strip_impulse_two_steps_closed_form(i0, j0):
+    """N = 1, impulse at step 0 inside (or at the edge of) the bottom, left and
+    right strips, including a corner; second step against the closed form."""
+    nz, nx, nB, N = 24, 30, 6, 1
+    dz, dx, dt = 10.0, 8.0, 1e-3
+    v = _velocity(nz, nx, 31)
+    _, s = oracle.fwd_cpml_2d(v, 2, N, nB, dz, dx, dt, src_iz=[i0], src_ix=[j0],
+                              wavelet=np.array([1.0, 0.0]), want_snapshot=True)
+    vd2 = (v * dt) ** 2
+    bx, bz = _b_profiles(v, nB, dz, dx, dt)
+    # the closed-form profile is the oracle's, to rounding
+    oxb, ozb = oracle.cpml_profiles_2d(v, nB, dz, dx, dt)
+    np.testing.assert_allclose(oxb, bx, rtol=1e-14)
+    np.testing.assert_allclose(ozb, bz, rtol=1e-14)
+    a = vd2[i0, j0]
+    p1 = np.zeros((nz, nx))
+    p1[i0, j0] = a
+    assert np.array_equal(s[:, :, 0], p1)
+    P = np.zeros((nz, nx))
+    bzc = bz[:, j0]
+    P[i0, j0] += -a * bzc[i0] * (bzc[i0] + (bzc[i0 - 1] if i0 > 0 else 1.0)) / dz ** 2
+    if i0 > 0:
+        P[i0 - 1, j0] += a * bzc[i0 - 1] ** 2 / dz ** 2
+    if i0 + 1 < nz:
+        P[i0 + 1, j0] += a * bzc[i0 + 1] * bzc[i0] / dz ** 2
+    bxr = bx[i0, :]
+    P[i0, j0] += -a * bxr[j0] * (bxr[j0] + (bxr[j0 - 1] if j0 > 0 else 1.0)) / dx ** 2
+    if j0 > 0:
+        P[i0, j0 - 1] += a * bxr[j0 - 1] ** 2 / dx ** 2
+    if j0 + 1 < nx:
+        P[i0, j0 + 1] += a * bxr[j0 + 1] * bxr[j0] / dx ** 2
+    p2 = vd2 * P + 2 * p1
+    np.testing.assert_allclose(s[:, :, 1], p2, rtol=1e-13, atol=1e-13 * abs(a) * 1e-6)
+    # the strip really varies across the impulse's neighbours, so b(h) and b(h+1)
+    # give different answers
+    if i0 >= nz - nB:
+        assert bzc[i0 - 1] != bzc[i0]
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5])
+@pytest.mark.parametrize("side", ["bottom", "left", "right"])
+def test_cpml_strip_onset_is_exact(N, side):
+    """Before any damped memory variable sees the wave, the CPML run is the
+    undamped run bit for bit; the first difference appears exactly at the step
+    and at the rows / columns that the damped points' stencil reach predicts.
+
+    Damped half points h+1/2 are those with b(h) != 1 (bottom h >= nz-nB, left
+    h <= nB-1, right h >= nx-nB); Phi(h+1/2) reads p over h-N+1 .. h+N, and
+    through A it changes D-A at i in h-N+1 .. h+N.  Damped integer points i
+    carry Psi(i), which reads p over i-L .. i+L (the D-D+ reach, L = 2N-1) and
+    changes P at i alone.  Snapshot t holds the field after step t, which
+    reaches L*t cells from the source, so a path that reads p at distance D
+    from the source first acts in snapshot ceil(D/L) + 1.  With b(h+1) in
+    place of b(h) at half point h+1/2 the Phi path moves by one cell and this
+    prediction fails."""
+    L = 2 * N - 1
+    nB, nt, gap = 6, 8, 4 * L
+    far = L * nt + nB + N + 2  # farther than any reach in nt steps
+    if side == "bottom":
+        iz = 1
+        nz = iz + gap + N - 1 + nB
+        nx = 2 * far + 1
+        jx = nx // 2
+        zb0 = nz - nB
+        paths = [(zb0 - L - iz, zb0),              # Psi(zb0): changes row zb0
+                 (zb0 - N + 1 - iz, zb0 - N + 1)]  # Phi(zb0+1/2): rows >= zb0-N+1
+    elif side == "left":
+        jx = nB + N - 1 + gap
+        nx = jx + far
+        iz = L * nt + 1
+        nz = iz + far
+        paths = [(jx - (nB - 1 + L), nB - 1), (jx - (nB - 1 + N), nB - 1 + N)]
+    else:
+        iz = L * nt + 1
+        nz = iz + far
+        jx = far
+        nx = jx + gap + nB + N - 1
+        paths = [(nx - nB - L - jx, nx - nB), (nx - nB - N + 1 - jx, nx - nB - N + 1)]
+    v = np.full((nz, nx), 2000.0)
+    args = dict(src_iz=[iz], src_ix=[jx], wavelet=np.ones(nt), want_snapshot=True)
+    _, sd = oracle.fwd_cpml_2d(v, nt, N, nB, 10.0, 10.0, 1e-3, **args)
+    _, sf = oracle.fwd_cpml_2d(v, nt, N, 0, 10.0, 10.0, 1e-3, **args)
+    first = [(-(-dist // L) + 1, edge) for dist, edge in paths]
+    t1 = min(t for t, _ in first)
+    assert 2 <= t1 < nt
+    for t in range(t1):
+        assert np.array_equal(sd[:, :, t], sf[:, :, t]), t
+    diff = np.argwhere(sd[:, :, t1] != sf[:, :, t1])
+    assert diff.size
+    edges = [e for t, e in first if t == t1]
+    if side == "bottom":
+        assert diff[:, 0].min() == min(edges)
+    elif side == "left":
+        assert diff[:, 1].max() == max(edges)
+    else:
+        assert diff[:, 1].min() == min(edges)
+
+
+def _sponge_g(nB):
+    # fwdTimeSpongeFor2dAw.m:47-50: boundary(m) = exp(-(0.015 (nB - m + 1))^2)^10, m = 1..nB
+    return np.array([math.exp(-(0.015 * (nB - m)) ** 2) ** 10 for m in range(nB)])
+
+
+def _sponge_factor(nz, nx, nB):
+    """Cerjan factor per cell (fwdTimeSpongeFor2dAw.m:83-92): side columns only in
+    rows 0..nz-nB-1, the bottom rows' factor across every column (corners get
+    the bottom factor alone)."""
+    g = _sponge_g(nB)
+    f = np.ones((nz, nx))
+    for i in range(nz - nB):
+        for j in range(nB):
+            f[i, j] = g[j]
+            f[i, nx - 1 - j] = g[j]
+    for i in range(nz - nB, nz):
+        f[i, :] = g[nz - 1 - i]
+    return f
+
+
+@pytest.mark.parametrize("cell", [(3, 0), (13, 0), (14, 2), (19, 0), (19, 23), (14, 21),
+                                  (13, 23), (16, 11), (5, 11)])
+def test_sponge_region_one_and_two_steps_closed_form(cell):
+    """Impulse in a side strip, at the last row that has side damping, in the
+    first bottom row, in corners, in the bottom strip and in the interior.
+    Step 1: p1 = -f v^2dt^2 at the cell (f = the cell's factor).  Step 2: the
+    neighbours get f(n) a(n) p1(c) and the cell f(c) b(c) p1(c): p1 is damped
+    once more as the previous level, which is exactly what the update reads
+    as p_old -- zero here -- so only the current level matters."""
+    nz, nx, nB = 20, 24, 6
+    i0, j0 = cell
+    v = _velocity(nz, nx, 41)
+    dt, dz, dx = 1e-3, 10.0, 9.0
+    _, s = oracle.fwd_sponge_2d(v, 3, nB, dz, dx, dt, src_iz=[i0], src_ix=[j0],
+                                wavelet=np.array([1.0, 0.0, 0.0]), want_snapshot=True)
+    f = _sponge_factor(nz, nx, nB)
+    vd2 = (v * dt) ** 2
+    ax = (v * dt / dx) ** 2
+    az = (v * dt / dz) ** 2
+    p1 = np.zeros((nz, nx))
+    p1[i0, j0] = -vd2[i0, j0] * f[i0, j0]
+    np.testing.assert_allclose(s[:, :, 0], p1, rtol=1e-15, atol=0)
+    p2 = np.zeros((nz, nx))
+    p2[i0, j0] = f[i0, j0] * (2 - 2 * ax[i0, j0] - 2 * az[i0, j0]) * p1[i0, j0]
+    for di, dj, a in [(0, 1, ax), (0, -1, ax), (1, 0, az), (-1, 0, az)]:
+        i, j = i0 + di, j0 + dj
+        if 0 <= i < nz and 0 <= j < nx:
+            p2[i, j] = f[i, j] * a[i, j] * p1[i0, j0]
+    np.testing.assert_allclose(s[:, :, 1], p2, rtol=1e-13, atol=1e-13 * abs(p1).max() * 1e-3)
+    # step 3 at the cell: the previous level enters as p_old with its second
+    # factor, p3(c) = f(c) [b(c) p2(c) - f(c) p1(c) + a-weighted neighbours]
+    i, j = i0, j0
+    nb = 0.0
+    for di, dj, a in [(0, 1, ax), (0, -1, ax), (1, 0, az), (-1, 0, az)]:
+        ii, jj = i + di, j + dj
+        if 0 <= ii < nz and 0 <= jj < nx:
+            nb += a[i, j] * p2[ii, jj]
+    p3c = f[i, j] * ((2 - 2 * ax[i, j] - 2 * az[i, j]) * p2[i, j] - f[i, j] * p1[i, j] + nb)
+    np.testing.assert_allclose(s[i, j, 2], p3c, rtol=1e-12)
+
+
+@pytest.mark.parametrize("nB,nt", [(0, 9), (5, 31), (7, 2), (4, 1)])
+def test_rvs_sponge_equals_fwd_with_reversed_surface_data(nB, nt):
+    """rvsTimeSpongeFor2dAw.m:69-111 is the forward sponge scheme run for
+    it = nt..1 with source(1,:) = data(:,it)' (same -v^2dt^2 sign): snapshot it
+    is the forward snapshot nt-it+1 of the dense source s(0,:,t) = data(:,nt-1-t),
+    and model is the previous level, which after the loop carries the
+    damping factor once more than the snapshot it was stored as."""
+    nz, nx = 17, 23
+    v = _velocity(nz, nx, 50 + nB)
+    data = np.random.default_rng(nt).standard_normal((nx, nt))
+    model, snap = oracle.rvs_sponge_2d(v, data, nB, 10.0, 9.0, 1e-3, want_snapshot=True)
+    src = np.zeros((nz, nx, nt))
+    src[0, :, :] = data[:, ::-1]
+    _, fsnap = oracle.fwd_sponge_2d(v, nt, nB, 10.0, 9.0, 1e-3, source=src, want_snapshot=True)
+    assert np.array_equal(snap, fsnap[:, :, ::-1])
+    want = np.zeros((nz + 2, nx + 2))
+    if nt >= 2:
+        want[1:-1, 1:-1] = _sponge_factor(nz, nx, nB) * fsnap[:, :, nt - 2]
+    assert np.array_equal(model, want)
+
+
+def _b_profiles_3d(v, nB, dt):
+    """src/fwdTimeCpmlFor3dAw.m:46-70 in closed form: dampPml with u in grid
+    index units and L = nB; x from the first y slice v(:, ., 1), y from the top
+    row v(1, ., .) replicated over z, z from v(., ., 1) replicated over y."""
+    nz, nx, ny = v.shape
+    lr = math.log(1e-6)
+    d = lambda vv, u: -(3 * vv) / (2 * nB) * lr * (u / nB) ** 2
+    bx, by, bz = np.ones(v.shape), np.ones(v.shape), np.ones(v.shape)
+    for q in range(nB):
+        bx[:, q, :] = np.exp(-d(v[:, q, 0], nB - q) * dt)[:, None]
+        bx[:, nx - nB + q, :] = np.exp(-d(v[:, nx - nB + q, 0], q + 1) * dt)[:, None]
+        by[:, :, q] = np.exp(-d(v[0, :, q], nB - q) * dt)[None, :]
+        by[:, :, ny - nB + q] = np.exp(-d(v[0, :, ny - nB + q], q + 1) * dt)[None, :]
+        bz[nz - nB + q, :, :] = np.exp(-d(v[nz - nB + q, :, 0], q + 1) * dt)[:, None]
+    return bx, by, bz
+
+
+@pytest.mark.parametrize("cell", [(5, 6, 2), (5, 6, 12), (9, 6, 1), (4, 1, 13), (8, 12, 7)])
+def test_cpml_3d_strip_impulse_two_steps_closed_form(cell):
+    """The 2-D impulse check in 3-D, N = 1, in the y (front / rear), x and z
+    strips and a corner: each axis contributes b(h) at half point h+1/2 and
+    b(i) at the integer point, with the 3-D files' index-unit profiles."""
+    nz, nx, ny, nB = 11, 14, 15, 4
+    i0, j0, k0 = cell
+    dz, dx, dy, dt = 10.0, 9.0, 8.0, 1e-3
+    v = np.random.default_rng(61).uniform(1500, 3000, (nz, nx, ny))
+    _, s = oracle.fwd_cpml_3d(v, 2, 1, nB, dz, dx, dy, dt, src_iz=[i0], src_ix=[j0],
+                              src_iy=[k0], wavelet=np.array([1.0, 0.0]), want_snapshot=True)
+    bx, by, bz = _b_profiles_3d(v, nB, dt)
+    oxb, oyb, ozb = oracle.cpml_profiles_3d(v, nB, dt)
+    np.testing.assert_allclose(oxb, bx, rtol=1e-14)
+    np.testing.assert_allclose(oyb, by, rtol=1e-14)
+    np.testing.assert_allclose(ozb, bz, rtol=1e-14)
+    vd2 = (v * dt) ** 2
+    a = vd2[i0, j0, k0]
+    p1 = np.zeros(v.shape)
+    p1[i0, j0, k0] = a
+    assert np.array_equal(s[..., 0], p1)
+    P = np.zeros(v.shape)
+    for axis, b, d in ((0, bz[:, j0, k0], dz), (1, bx[i0, :, k0], dx), (2, by[i0, j0, :], dy)):
+        c = [i0, j0, k0][axis]
+        n = v.shape[axis]
+
+        def at(off):
+            idx = [i0, j0, k0]
+            idx[axis] = c + off
+            return tuple(idx)
+        P[at(0)] += -a * b[c] * (b[c] + (b[c - 1] if c > 0 else 1.0)) / d ** 2
+        if c > 0:
+            P[at(-1)] += a * b[c - 1] ** 2 / d ** 2
+        if c + 1 < n:
+            P[at(1)] += a * b[c + 1] * b[c] / d ** 2
+    np.testing.assert_allclose(s[..., 1], vd2 * P + 2 * p1, rtol=1e-11, atol=1e-12 * abs(a))
