@@ -31,18 +31,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 def workload():
-    from sssi_amd import synth
-    nz0, nx0, nB, N = 1000, 3000, 40, 5
-    dz = dx = 5.0
-    dt = 5e-4
-    nt = 4000
-    v = synth.layered_2d(nz0, nx0, 20, 1500.0, 5000.0, seed=2)
-    v = v * (1.0 + synth.gaussian_perturbation(v.shape, 0.05, 10.0, seed=3))
-    v = np.clip(v, 1500.0, 5000.0)
-    V = synth.extBoundary(v, nB, 2)
-    w = synth.ricker(25.0, nt, dt)
-    return dict(V=V, nB=nB, N=N, dz=dz, dx=dx, dt=dt, nt=nt, wavelet=w, src_iz=[0],
-                src_ix=[nx0 // 2 + nB])
+    from sssi_amd import workloads
+    return workloads.config2()
 
 
 def traffic_from_profiles(name="cpml2d_step_bench_traffic.json"):
@@ -71,18 +61,8 @@ def cpu_baseline(wl, steps=60):
 
 
 def workload_3d():
-    from sssi_amd import synth
-    n0, nB, N = 400, 20, 4
-    d = 10.0
-    dt = 1e-3
-    nt = 2000
-    v = synth.cake_3d_dipping(n0, n0, n0, 8, 1500.0, 4500.0, seed=4)
-    # the 8th-order 3-D scheme at dt = 1 ms, d = 10 m is stable below 4490 m/s
-    v = np.clip(v, 1500.0, 4450.0)
-    V = synth.extBoundary(v, nB, 3)
-    w = synth.ricker(15.0, nt, dt)
-    c = (n0 + 2 * nB) // 2
-    return dict(V=V, nB=nB, N=N, d=d, dt=dt, nt=nt, wavelet=w, src=(0, c, c))
+    from sssi_amd import workloads
+    return workloads.config4()
 
 
 def bench_3d(args):
@@ -172,19 +152,8 @@ def bench_sponge(args):
 
 
 def workload_rtm(nshots=64, seed=5):
-    from sssi_amd import synth
-    nz, nx, nB, N = 600, 6000, 30, 4
-    d, dt, nt = 10.0, 1e-3, 3000
-    V = synth.layered_2d(nz, nx, 12, 1500.0, 4500.0, seed=seed)
-    VS = synth.smooth_copy(V, 10)
-    rng = np.random.default_rng(seed + 1)
-    w = synth.ricker(15.0, nt, dt)
-    shots = []
-    for _ in range(nshots):
-        xc = int(rng.integers(400, 1201))
-        xb = int(rng.integers(0, nx - xc + 1))
-        shots.append((xb, xc, xc // 2))
-    return dict(V=V, VS=VS, nB=nB, N=N, d=d, dt=dt, nt=nt, wavelet=w, shots=shots)
+    from sssi_amd import workloads
+    return workloads.config3(nshots, seed)
 
 
 def bench_rtm(args):
@@ -221,39 +190,8 @@ def bench_rtm(args):
 
 
 def serve_jobs(njobs, seed=7):
-    """Seeded job population: 70 % 2-D (forward or RTM shot), grids 200x200 ..
-    2000x4000, nDiffOrder 2..5, nt 500..3000; 30 % 3-D forward, 128^3 .. 256^3,
-    nt 500..1500; layered 1500-4500 m/s; Ricker 10-30 Hz."""
-    from sssi_amd import synth
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for k in range(njobs):
-        f = float(rng.uniform(10, 30))
-        if rng.random() < 0.7:
-            nz, nx = int(rng.integers(200, 2001)), int(rng.integers(200, 4001))
-            N = int(rng.integers(2, 6))
-            nt = int(rng.integers(500, 3001))
-            v = synth.layered_2d(nz, nx, 6, 1500.0, 4500.0, seed=seed * 1000 + k)
-            nB = 20
-            dt = 0.5 * 10.0 / (4500.0 * np.sqrt(2) * float(np.sum(np.abs(synth.dCoef(N)))))
-            w = synth.ricker(f, nt, dt)
-            if rng.random() < 0.5:
-                jobs.append(("fwd2d", dict(v=v, nt=nt, nDiffOrder=N, nBoundary=nB, dz=10.0,
-                                           dx=10.0, dt=dt, src_iz=0, src_ix=nx // 2, wavelet=w)))
-            else:
-                jobs.append(("rtm2d", dict(V=v, VS=synth.smooth_copy(v, 5), nt=nt, nDiffOrder=N,
-                                           nBoundary=nB, dz=10.0, dx=10.0, dt=dt, src_ix=nx // 2,
-                                           wavelet=w, img_stride=5, ck_format="bf16")))
-        else:
-            n = int(rng.integers(128, 257))
-            nt = int(rng.integers(500, 1501))
-            N = int(rng.integers(2, 6))
-            v = synth.cake_3d_dipping(n, n, n, 4, 1500.0, 4500.0, seed=seed * 1000 + k)
-            dt = 0.5 * 10.0 / (4500.0 * np.sqrt(3) * float(np.sum(np.abs(synth.dCoef(N)))))
-            jobs.append(("fwd3d", dict(v=v, nt=nt, nDiffOrder=N, nBoundary=16, dz=10.0, dx=10.0,
-                                       dy=10.0, dt=dt, src=(0, n // 2, n // 2),
-                                       wavelet=synth.ricker(f, nt, dt))))
-    return jobs
+    from sssi_amd import workloads
+    return workloads.config5(njobs, seed)
 
 
 def bench_serve(args):

@@ -126,6 +126,7 @@ void run_oneshot(Prop &P, int64_t nt, bool reverse, double *data, double *snap, 
             done += count;
         }
     }
+    P.scan_field();
     P.sync();
     if (data) P.get_data(data, nt);
 }
@@ -161,12 +162,14 @@ int sssi_fwd_cpml_2d_points(const double *v, int64_t nz, int64_t nx, int64_t nt,
                             double *data, double *snapshot, int64_t snap_stride) {
     return guarded([&] {
         need(v, "v");
-        validate_common(nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_common(2, nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_velocity(v, nz * nx, "v");
         if (snapshot && snap_stride < 1) fail(SSSI_EINVAL, "snap_stride must be >= 1");
         std::unique_ptr<Prop> P(make_prop2d(v, nz, nx, SSSI_ABC_CPML, nDiffOrder, nBoundary, dz,
                                             dx, dt));
         P->set_points(nsrc, src_iz, src_ix, nullptr, wavelet, nt);
         run_oneshot(*P, nt, false, data, snapshot, snapshot ? snap_stride : 1);
+        P->check_health();
     });
 }
 
@@ -176,12 +179,14 @@ int sssi_fwdTimeCpmlFor2dAw(const double *v, int64_t nz, int64_t nx, const doubl
     return guarded([&] {
         need(v, "v");
         need(source, "source");
-        validate_common(nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_common(2, nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_velocity(v, nz * nx, "v");
         PointList pl = dense_to_points(source, nz, nx, 1, nt);
         std::unique_ptr<Prop> P(make_prop2d(v, nz, nx, SSSI_ABC_CPML, nDiffOrder, nBoundary, dz,
                                             dx, dt));
         P->set_points((int)pl.iz.size(), pl.iz.data(), pl.ix.data(), nullptr, pl.w.data(), nt);
         run_oneshot(*P, nt, false, data, snapshot, 1);
+        P->check_health();
     });
 }
 
@@ -191,13 +196,15 @@ int sssi_rvs_cpml_2d_ex(const double *v, int64_t nz, int64_t nx, const double *d
     return guarded([&] {
         need(v, "v");
         need(data, "data");
-        validate_common(nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_common(2, nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_velocity(v, nz * nx, "v");
         if (snapshot && snap_stride < 1) fail(SSSI_EINVAL, "snap_stride must be >= 1");
         std::unique_ptr<Prop> P(make_prop2d(v, nz, nx, SSSI_ABC_CPML, nDiffOrder, nBoundary, dz,
                                             dx, dt));
         P->set_surface(data, nt);
         run_oneshot(*P, nt, true, nullptr, snapshot, snapshot ? snap_stride : 1);
         if (model) P->get_model(model);
+        P->check_health();
     });
 }
 
@@ -214,11 +221,14 @@ int sssi_fwd_sponge_2d_points(const double *v, int64_t nz, int64_t nx, int64_t n
                               double *snapshot, int64_t snap_stride) {
     return guarded([&] {
         need(v, "v");
-        validate_common(nz, nx, 1, nt, 1, nBoundary, dz, dx, 1.0, dt, false);
+        validate_common(2, nz, nx, 1, nt, 1, nBoundary, dz, dx, 1.0, dt, false);
+        validate_sponge_grid(nz, nx);
+        validate_velocity(v, nz * nx, "v");
         if (snapshot && snap_stride < 1) fail(SSSI_EINVAL, "snap_stride must be >= 1");
         std::unique_ptr<Prop> P(make_prop2d(v, nz, nx, SSSI_ABC_SPONGE, 1, nBoundary, dz, dx, dt));
         P->set_points(nsrc, src_iz, src_ix, nullptr, wavelet, nt);
         run_oneshot(*P, nt, false, data, snapshot, snapshot ? snap_stride : 1);
+        P->check_health();
     });
 }
 
@@ -228,11 +238,14 @@ int sssi_fwdTimeSpongeFor2dAw(const double *v, int64_t nz, int64_t nx, const dou
     return guarded([&] {
         need(v, "v");
         need(source, "source");
-        validate_common(nz, nx, 1, nt, 1, nBoundary, dz, dx, 1.0, dt, false);
+        validate_common(2, nz, nx, 1, nt, 1, nBoundary, dz, dx, 1.0, dt, false);
+        validate_sponge_grid(nz, nx);
+        validate_velocity(v, nz * nx, "v");
         PointList pl = dense_to_points(source, nz, nx, 1, nt);
         std::unique_ptr<Prop> P(make_prop2d(v, nz, nx, SSSI_ABC_SPONGE, 1, nBoundary, dz, dx, dt));
         P->set_points((int)pl.iz.size(), pl.iz.data(), pl.ix.data(), nullptr, pl.w.data(), nt);
         run_oneshot(*P, nt, false, data, snapshot, 1);
+        P->check_health();
     });
 }
 
@@ -242,12 +255,15 @@ int sssi_rvs_sponge_2d_ex(const double *v, int64_t nz, int64_t nx, const double 
     return guarded([&] {
         need(v, "v");
         need(data, "data");
-        validate_common(nz, nx, 1, nt, 1, nBoundary, dz, dx, 1.0, dt, false);
+        validate_common(2, nz, nx, 1, nt, 1, nBoundary, dz, dx, 1.0, dt, false);
+        validate_sponge_grid(nz, nx);
+        validate_velocity(v, nz * nx, "v");
         if (snapshot && snap_stride < 1) fail(SSSI_EINVAL, "snap_stride must be >= 1");
         std::unique_ptr<Prop> P(make_prop2d(v, nz, nx, SSSI_ABC_SPONGE, 1, nBoundary, dz, dx, dt));
         P->set_surface(data, nt);
         run_oneshot(*P, nt, true, nullptr, snapshot, snapshot ? snap_stride : 1);
         if (model) P->get_model(model);
+        P->check_health();
     });
 }
 
@@ -264,11 +280,13 @@ int sssi_fwd_cpml_3d_points(const double *v, int64_t nz, int64_t nx, int64_t ny,
                             int64_t snap_stride) {
     return guarded([&] {
         need(v, "v");
-        validate_common(nz, nx, ny, nt, nDiffOrder, nBoundary, dz, dx, dy, dt, true);
+        validate_common(3, nz, nx, ny, nt, nDiffOrder, nBoundary, dz, dx, dy, dt, true);
+        validate_velocity(v, nz * nx * ny, "v");
         if (snapshot && snap_stride < 1) fail(SSSI_EINVAL, "snap_stride must be >= 1");
         std::unique_ptr<Prop> P(make_prop3d(v, nz, nx, ny, nDiffOrder, nBoundary, dz, dx, dy, dt));
         P->set_points(nsrc, src_iz, src_ix, src_iy, wavelet, nt);
         run_oneshot(*P, nt, false, data, snapshot, snapshot ? snap_stride : 1);
+        P->check_health();
     });
 }
 
@@ -279,12 +297,14 @@ int sssi_fwdTimeCpmlFor3dAw(const double *v, int64_t nz, int64_t nx, int64_t ny,
     return guarded([&] {
         need(v, "v");
         need(source, "source");
-        validate_common(nz, nx, ny, nt, nDiffOrder, nBoundary, dz, dx, dy, dt, true);
+        validate_common(3, nz, nx, ny, nt, nDiffOrder, nBoundary, dz, dx, dy, dt, true);
+        validate_velocity(v, nz * nx * ny, "v");
         PointList pl = dense_to_points(source, nz, nx, ny, nt);
         std::unique_ptr<Prop> P(make_prop3d(v, nz, nx, ny, nDiffOrder, nBoundary, dz, dx, dy, dt));
         P->set_points((int)pl.iz.size(), pl.iz.data(), pl.ix.data(), pl.iy.data(), pl.w.data(),
                       nt);
         run_oneshot(*P, nt, false, data, snapshot, 1);
+        P->check_health();
     });
 }
 
@@ -295,8 +315,10 @@ int sssi_prop2d_create(const double *v, int64_t nz, int64_t nx, int abc, int nDi
         need(v, "v");
         need(out, "out");
         if (abc != SSSI_ABC_CPML && abc != SSSI_ABC_SPONGE) fail(SSSI_EINVAL, "unknown abc %d", abc);
-        validate_common(nz, nx, 1, 1, nDiffOrder, nBoundary, dz, dx, 1.0, dt,
+        validate_common(2, nz, nx, 1, 1, nDiffOrder, nBoundary, dz, dx, 1.0, dt,
                         abc == SSSI_ABC_CPML);
+        if (abc == SSSI_ABC_SPONGE) validate_sponge_grid(nz, nx);
+        validate_velocity(v, nz * nx, "v");
         auto h = std::make_unique<sssi_prop>();
         h->impl.reset(make_prop2d(v, nz, nx, abc, abc == SSSI_ABC_CPML ? nDiffOrder : 1,
                                   nBoundary, dz, dx, dt));
@@ -310,7 +332,8 @@ int sssi_prop3d_create(const double *v, int64_t nz, int64_t nx, int64_t ny, int 
     return guarded([&] {
         need(v, "v");
         need(out, "out");
-        validate_common(nz, nx, ny, 1, nDiffOrder, nBoundary, dz, dx, dy, dt, true);
+        validate_common(3, nz, nx, ny, 1, nDiffOrder, nBoundary, dz, dx, dy, dt, true);
+        validate_velocity(v, nz * nx * ny, "v");
         auto h = std::make_unique<sssi_prop>();
         h->impl.reset(make_prop3d(v, nz, nx, ny, nDiffOrder, nBoundary, dz, dx, dy, dt));
         *out = h.release();
@@ -374,24 +397,31 @@ int sssi_prop_run_timed(sssi_prop *p, int64_t nt, int reverse, int record, doubl
         (void)hipEventDestroy(e1);
         *ms = f;
         if (launches) *launches = st.launches;
+        P.check_health();
     });
 }
 
 int sssi_prop_sync(sssi_prop *p) {
-    return guarded([&] { impl(p).sync(); });
+    return guarded([&] {
+        impl(p).sync();
+        impl(p).check_health();
+    });
 }
 
 int sssi_prop_get_data(sssi_prop *p, double *data, int64_t nt) {
     return guarded([&] {
         need(data, "data");
         impl(p).get_data(data, nt);
+        impl(p).check_health();
     });
 }
 
 int sssi_prop_get_field(sssi_prop *p, double *field) {
     return guarded([&] {
         need(field, "field");
+        impl(p).scan_field();
         impl(p).get_field(field);
+        impl(p).check_health();
     });
 }
 

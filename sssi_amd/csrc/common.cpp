@@ -1,6 +1,7 @@
 // common.cpp -- host helpers shared by the propagators (see common.h).
 #include "common.h"
 
+#include <atomic>
 #include <cstring>
 
 namespace sssi {
@@ -82,8 +83,8 @@ void cpml2d_bm1(const double *v, int64_t nz, int64_t nx, int nB, double dz, doub
     }
 }
 
-void validate_common(int64_t nz, int64_t nx, int64_t ny, int64_t nt, int N, int nB, double dz,
-                     double dx, double dy, double dt, bool need_N) {
+void validate_common(int dim, int64_t nz, int64_t nx, int64_t ny, int64_t nt, int N, int nB,
+                     double dz, double dx, double dy, double dt, bool need_N) {
     if (nz < 1 || nx < 1 || ny < 1)
         fail(SSSI_EINVAL, "grid must be non-empty (nz=%lld nx=%lld ny=%lld)", (long long)nz,
              (long long)nx, (long long)ny);
@@ -95,11 +96,28 @@ void validate_common(int64_t nz, int64_t nx, int64_t ny, int64_t nt, int N, int 
     if ((int64_t)nB > nz) fail(SSSI_EINVAL, "nBoundary=%d exceeds nz=%lld", nB, (long long)nz);
     if (2 * (int64_t)nB > nx)
         fail(SSSI_EINVAL, "2*nBoundary=%d exceeds nx=%lld", 2 * nB, (long long)nx);
-    if (ny > 1 && 2 * (int64_t)nB > ny)
+    if ((dim == 3 || ny > 1) && 2 * (int64_t)nB > ny)
         fail(SSSI_EINVAL, "2*nBoundary=%d exceeds ny=%lld", 2 * nB, (long long)ny);
     if (!(dz > 0) || !(dx > 0) || !(dy > 0) || !(dt > 0) || !std::isfinite(dz) ||
         !std::isfinite(dx) || !std::isfinite(dy) || !std::isfinite(dt))
         fail(SSSI_EINVAL, "dz, dx, dy and dt must be positive and finite");
+}
+
+void validate_velocity(const double *v, int64_t n, const char *name) {
+    if (!v) fail(SSSI_EINVAL, "%s must not be NULL", name);
+    std::atomic<int64_t> bad{-1};
+    parallel_ranges(n, [&](int64_t a, int64_t b) {
+        for (int64_t q = a; q < b; ++q)
+            if (!(v[q] > 0.0) || !std::isfinite(v[q])) {
+                int64_t want = -1;
+                bad.compare_exchange_strong(want, q);
+                return;
+            }
+    }, 1 << 16);
+    const int64_t q = bad.load();
+    if (q >= 0)
+        fail(SSSI_EINVAL, "%s must be finite and positive (element %lld is %g)", name,
+             (long long)q, v[q]);
 }
 
 }  // namespace sssi

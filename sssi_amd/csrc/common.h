@@ -129,7 +129,12 @@ void cpml2d_bm1(const double *v, int64_t nz, int64_t nx, int nB, double dz, doub
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
-void validate_common(int64_t nz, int64_t nx, int64_t ny, int64_t nt, int N, int nB, double dz,
-                     double dx, double dy, double dt, bool need_N);
+// Argument checks shared by every entry point (before any device work).
+// dim 3 enforces 2*nB <= ny also for ny = 1 (a 3-D call with a 2-D model).
+void validate_common(int dim, int64_t nz, int64_t nx, int64_t ny, int64_t nt, int N, int nB,
+                     double dz, double dx, double dy, double dt, bool need_N);
+// Every velocity finite and > 0 (the reference accepts anything and returns
+// NaN/Inf traces); scanned on up to 16 host threads.
+void validate_velocity(const double *v, int64_t n, const char *name);
 
 }  // namespace sssi

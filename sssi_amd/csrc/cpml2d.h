@@ -7,6 +7,7 @@
 #include <functional>
 #include <vector>
 
+#include "health.h"
 #include "prop.h"
 
 namespace sssi {
@@ -57,6 +58,9 @@ struct Job2D {
     const float *im_scale;
     int im_stride, im_fmt;
     float *im_num, *im_den;
+    // non-finite watchdog word (kHealthy, or the first launch step whose top
+    // row held a NaN/Inf); null: no check
+    int *health;
 };
 
 // Per-model device state of a 2-D CPML propagation (velocity-dependent
@@ -71,6 +75,7 @@ class Cpml2dGrid {
     double dz = 0, dx = 0, dt = 0;
     DevBuf<float> p[2], vdt2;
     DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1;
+    DevBuf<int> health;  // non-finite watchdog word of the jobs on this grid
     float cz[8] = {0}, cx[8] = {0}, w0 = 0.f, w0lo = 0.f, wz[16] = {0}, wx[16] = {0};
     float w0z = 0.f, w0zlo = 0.f, w0x = 0.f, w0xlo = 0.f;
 
@@ -90,7 +95,10 @@ class Cpml2dGrid {
     // tile block covered by the interior march (not launched as a tile block)
     bool block_in_march(int tile) const;
     int n_march_blocks() const { return mntx > 0 ? (mz_rows / (4 * ng * rz)) * mntx : 0; }
+    // zero the wavefield and memory variables and re-arm the watchdog
     void zero_state(hipStream_t s);
+    // scan level `which` for NaN/Inf (reported into health as `tag`)
+    void scan_nonfinite(int which, int tag, hipStream_t s) const;
     // descriptor with geometry, coefficients and state pointers filled in
     Job2D job() const;
     // interior (nz x nx, x fastest) of level `which` -> column-major doubles

@@ -126,7 +126,7 @@ __device__ __forceinline__ void tile_update(const Job2D &J, const StepCtx &S,
                                             uint32_t srow, int zs, int j, bool jin,
                                             const float (&pc)[RZ], const float (&po)[RZ],
                                             const float (&v2)[RZ], const float (&lap)[RZ],
-                                            const float (&s)[RZ], float &wmax) {
+                                            const float (&s)[RZ], float &wmax, float &pn0) {
     const size_t ncell = (size_t)J.nz * J.nx;
 #pragma unroll
     for (int r = 0; r < RZ; ++r) {
@@ -137,6 +137,7 @@ __device__ __forceinline__ void tile_update(const Job2D &J, const StepCtx &S,
             // scale of the increment rather than of the field.
             const float pn = pc[r] + fmaf(v2[r], lap[r] + s[r], pc[r] - po[r]);
             buf_st(pn, rO, vcol, srow + r * S.rowb);
+            if (r == 0) pn0 = pn;
             const size_t cell = (size_t)i * J.nx + j;
             if (J.rec && i == 0) J.rec[(size_t)S.t * J.nx + j] = pn;
             if (S.snap_now)
@@ -468,9 +469,11 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
 #pragma unroll
     for (int r = 0; r < RZ; ++r) pc[r] = col[L + r];
     if (S.pack_now && tile == 0 && w == 0 && lane == 0) J.ck_scale[S.c - 1] = S.scale;
-    float wmax = 0.f;
-    tile_update<RZ>(J, S, rO, vcol, srow, zs, j, jin, pc, po, v2, lap, s, wmax);
+    float wmax = 0.f, pn0 = 0.f;
+    tile_update<RZ>(J, S, rO, vcol, srow, zs, j, jin, pc, po, v2, lap, s, wmax, pn0);
     ck_max_commit(J, S, wmax, tile * 4 + w, lane);
+    // watchdog on the top row (the receivers' row)
+    if (J.health && zs == 0) report_nonfinite(J.health, jin && nonfinite(pn0), step);
 }
 
 // Interior march: a wavefront owns 64 columns x NG*RZ rows of the interior
@@ -561,8 +564,10 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
             pog[r] = po[g * RZ + r];
             v2g[r] = v2[g * RZ + r];
         }
+        float pn0 = 0.f;
         tile_update<RZ>(J, S, rO, vcol, srow0 + g * RZ * rowb, zs, j, true, pc, pog, v2g, lap, s,
-                        wmax);
+                        wmax, pn0);
+        if (J.health && zs == 0) report_nonfinite(J.health, nonfinite(pn0), step);
     }
     if (S.pack_now && mb == 0 && w == 0 && lane == 0) J.ck_scale[S.c - 1] = S.scale;
     ck_max_commit(J, S, wmax, mb * 4 + w, lane);
@@ -668,6 +673,8 @@ Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB
     p[0].alloc(np);
     p[1].alloc(np);
     vdt2.alloc(np);
+    health.alloc(1);
+    SSSI_HIP(hipMemset(health.p, 0x7f, sizeof(int)));
     {
         // v (column-major, z fastest) -> padded x-fastest rows, zero halo.  The
         // zero fill and the transpose are split over host threads; the
@@ -791,6 +798,40 @@ int march_groups(int rz) {
 void Cpml2dGrid::zero_state(hipStream_t s) {
     for (auto *b : {&p[0], &p[1], &zphi[0], &zphi[1], &zpsi, &xphi[0], &xphi[1], &xpsi})
         if (b->p) SSSI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), s));
+    arm_health(health.p, s);
+}
+
+void Cpml2dGrid::scan_nonfinite(int which, int tag, hipStream_t s) const {
+    launch_nonfinite_scan(p[which].p, p[which].n, health.p, tag, s);
+}
+
+// Grid-stride scan: one atomic per wave that holds a NaN/Inf.
+__global__ void __launch_bounds__(256) nonfinite_scan(const float *__restrict__ p, size_t n,
+                                                      int *health, int tag) {
+    bool bad = false;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x)
+        bad |= nonfinite(p[q]);
+    report_nonfinite(health, bad, tag);
+}
+
+void launch_nonfinite_scan(const float *p, size_t n, int *health, int tag, hipStream_t s) {
+    if (!p || !health || n == 0) return;
+    const size_t blocks = std::min<size_t>(2048, (n + 255) / 256);
+    hipLaunchKernelGGL(nonfinite_scan, dim3((unsigned)blocks), dim3(256), 0, s, p, n, health, tag);
+    SSSI_HIP(hipGetLastError());
+}
+
+int read_health(const int *health, hipStream_t s) {
+    if (!health) return kHealthy;
+    int h = kHealthy;
+    SSSI_HIP(hipMemcpyAsync(&h, health, sizeof(int), hipMemcpyDeviceToHost, s));
+    SSSI_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+void arm_health(int *health, hipStream_t s) {
+    if (health) SSSI_HIP(hipMemsetAsync(health, 0x7f, sizeof(int), s));
 }
 
 Job2D Cpml2dGrid::job() const {
@@ -811,6 +852,7 @@ Job2D Cpml2dGrid::job() const {
     J.vdt2 = vdt2.p;
     J.zphi[0] = zphi[0].p; J.zphi[1] = zphi[1].p; J.zpsi = zpsi.p; J.zbm1 = zbm1.p;
     J.xphi[0] = xphi[0].p; J.xphi[1] = xphi[1].p; J.xpsi = xpsi.p; J.xbm1 = xbm1.p;
+    J.health = health.p;
     J.snap_stride = 1;
     J.ck_stride = 1;
     J.im_stride = 1;

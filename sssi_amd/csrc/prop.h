@@ -18,6 +18,21 @@ constexpr int kHX = 32;
 constexpr int kTX = 64;     // tile width (one wavefront across x)
 constexpr int kTZmax = 64;  // tallest tile the kernels use
 
+// Bytes of one padded 2-D level in the layout above.
+inline int64_t padded_bytes_2d(int64_t nz, int64_t nx) {
+    const int64_t pitch = round_up(kHX + round_up(nx, kTX) + 32, 64);
+    const int64_t rows = kHZ + round_up(nz, kTZmax) + kHZbot;
+    return pitch * rows * 4;
+}
+// The sponge step drops the stores of lanes outside the model by giving them
+// the out-of-range byte offset 2^31, which is out of range only for buffers
+// below 2 GiB: its padded levels are capped there.
+inline void validate_sponge_grid(int64_t nz, int64_t nx) {
+    if (padded_bytes_2d(nz, nx) >= ((int64_t)1 << 31))
+        fail(SSSI_EINVAL, "sponge grid %lldx%lld too large: a padded level needs %lld bytes "
+             "(limit 2 GiB)", (long long)nz, (long long)nx, (long long)padded_bytes_2d(nz, nx));
+}
+
 struct LaunchStats {
     int64_t launches = 0;
 };
@@ -70,6 +85,16 @@ class Prop {
     virtual void alg_bytes(double *bytes, double *cells) = 0;
 
     void sync() { SSSI_HIP(hipStreamSynchronize(stream)); }
+
+    // Non-finite watchdog.  The kernels report a NaN/Inf on the top (receiver)
+    // row into a device word every step; scan_field() also scans the whole
+    // current level.  check_health() waits for the stream and fails with
+    // SSSI_EUNSTABLE if either saw one since the last reset().
+    virtual int *health_word() const = 0;
+    virtual void scan_field() = 0;
+    int64_t steps_done = 0;  // steps since the last reset()
+    double courant = 0.0;    // vmax dt sqrt(sum 1/d^2), for the message
+    void check_health();
 
     // Grow the receiver buffer to nt steps of `per_step` values, keeping what
     // was recorded so far.

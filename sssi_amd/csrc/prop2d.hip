@@ -32,6 +32,8 @@ struct Sponge2dArgs {
     const float *surf;
     float *rec;
     float *snap;
+    int *health;  // non-finite watchdog word (top row), or null
+    int step;     // launch step reported to it
 };
 
 // Rows per wavefront of the sponge step; a block is 4 wavefronts stacked in z.
@@ -101,6 +103,7 @@ __global__ void __launch_bounds__(256) sponge2d_step(Sponge2dArgs a, int ntx) {
     }
     if (a.surf && i0 == 0 && jin) s[0] += a.surf[j];
 
+    float pn0 = 0.f;
 #pragma unroll
     for (int r = 0; r < RZ; ++r) {
         const int i = i0 + r;
@@ -116,11 +119,13 @@ __global__ void __launch_bounds__(256) sponge2d_step(Sponge2dArgs a, int ntx) {
         float pn = pc + ((pc - gf * po[r]) + (lap - v2 * s[r]));
         pn *= gf;
         buf_st(pn, rO, jin ? vcol : kOOB, srow + r * rowb);
+        if (r == 0) pn0 = pn;
         if (jin) {
             if (a.rec && i == 0) a.rec[j] = pn;
             if (a.snap) a.snap[(size_t)i * a.nx + j] = pn;
         }
     }
+    if (a.health && i0 == 0) report_nonfinite(a.health, jin && nonfinite(pn0), a.step);
 }
 
 // ---------------------------------------------------------------------------
@@ -136,6 +141,7 @@ class Prop2D final : public Prop {
     // sponge
     int64_t pitch = 0, rows = 0;
     DevBuf<float> sp[2], vdt, gtab;
+    DevBuf<int> sp_health;
     std::vector<int> h_cells;
     int cur = 0;
 
@@ -146,15 +152,20 @@ class Prop2D final : public Prop {
         nz = nz_; nx = nx_; ny = 1; N = N_; nB = nB_; dz = dz_; dx = dx_; dt = dt_;
         SSSI_HIP(hipGetDevice(&device));
         SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        {
+            double vmax = 0.0;
+            for (int64_t q = 0; q < nz * nx; ++q) vmax = std::max(vmax, v[q]);
+            courant = vmax * dt * std::sqrt(1.0 / (dz * dz) + 1.0 / (dx * dx));
+        }
         if (abc == SSSI_ABC_CPML) {
             g.reset(new Cpml2dGrid(v, nz, nx, N, nB, dz, dx, dt));
             if (autotune_enabled()) tune_layout();
             build_order();
         } else {
+            validate_sponge_grid(nz, nx);
             pitch = round_up(kHX + round_up(nx, kTX) + 32, 64);
             rows = kHZ + round_up(nz, kTZmax) + kHZbot;
             const size_t np = (size_t)pitch * rows;
-            if (np * 4 >= ((size_t)1 << 32)) fail(SSSI_EINVAL, "2-D padded grid exceeds 4 GiB");
             sp[0].alloc(np);
             sp[1].alloc(np);
             vdt.alloc(np);
@@ -179,6 +190,7 @@ class Prop2D final : public Prop {
             }
             gtab.alloc(gg.size());
             SSSI_HIP(hipMemcpy(gtab.p, gg.data(), gtab.bytes(), hipMemcpyHostToDevice));
+            sp_health.alloc(1);
         }
         reset();
     }
@@ -272,8 +284,10 @@ class Prop2D final : public Prop {
         } else {
             SSSI_HIP(hipMemsetAsync(sp[0].p, 0, sp[0].bytes(), stream));
             SSSI_HIP(hipMemsetAsync(sp[1].p, 0, sp[1].bytes(), stream));
+            arm_health(sp_health.p, stream);
         }
         done = 0;
+        steps_done = 0;
         cur = 0;
         SSSI_HIP(hipStreamSynchronize(stream));
     }
@@ -360,8 +374,17 @@ class Prop2D final : public Prop {
         return J;
     }
 
-    void step_sponge(int64_t t, bool reverse, bool record, float *snap) {
+    int *health_word() const override { return g ? g->health.p : sp_health.p; }
+
+    void scan_field() override {
+        if (g) g->scan_nonfinite((int)(done & 1), (int)done, stream);
+        else launch_nonfinite_scan(sp[cur].p, sp[cur].n, sp_health.p, (int)done, stream);
+    }
+
+    void step_sponge(int64_t t, bool reverse, bool record, float *snap, int step) {
         Sponge2dArgs a{};
+        a.health = sp_health.p;
+        a.step = step;
         a.pcur = sp[cur].p;
         a.pold = sp[cur ^ 1].p;
         a.vdt = vdt.p;
@@ -399,6 +422,7 @@ class Prop2D final : public Prop {
                       });
             if (st) st->launches += count;
             done += count;
+            steps_done = done;
             return;
         }
         for (int64_t q = 0; q < count; ++q) {
@@ -406,10 +430,11 @@ class Prop2D final : public Prop {
             float *snap = nullptr;
             if (sink && sink->dev && t % sink->stride == 0)
                 snap = sink->dev + (size_t)(t / sink->stride - sink->slot0) * nz * nx;
-            step_sponge(t, reverse, record, snap);
+            step_sponge(t, reverse, record, snap, (int)(done + q));
             if (st) st->launches++;
         }
         done += count;
+        steps_done = done;
     }
 
     void get_data(double *data, int64_t nt) override {
@@ -474,6 +499,15 @@ class Prop2D final : public Prop {
         *bytes = g ? g->alg_bytes_per_step() : 16.0 * (double)nz * nx + 4.0 * nx;
     }
 };
+
+void Prop::check_health() {
+    const int h = read_health(health_word(), stream);
+    if (h != kHealthy)
+        fail(SSSI_EUNSTABLE,
+             "non-finite wavefield (NaN/Inf) at step %d of %lld since the last reset: the time "
+             "step is likely beyond the scheme's stability limit (v_max dt sqrt(sum 1/d^2) = %.3f)",
+             h + 1, (long long)steps_done, courant);
+}
 
 Prop *make_prop2d(const double *v, int64_t nz, int64_t nx, int abc, int N, int nB, double dz,
                   double dx, double dt) {

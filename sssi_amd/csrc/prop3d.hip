@@ -26,6 +26,7 @@
 #include <numeric>
 #include <vector>
 
+#include "health.h"
 #include "prop.h"
 #include "tune.h"
 
@@ -62,6 +63,8 @@ struct Cpml3dArgs {
     int nsrc;
     float *rec;   // this step's receivers [k*nx + j] or null
     float *snap;  // this step's snapshot [(i*ny + k)*nx + j] or null
+    int *health;  // non-finite watchdog word, or null
+    int step;     // launch step reported to it
 };
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc3(const void *base, uint32_t bytes) {
@@ -260,6 +263,7 @@ __device__ __forceinline__ void cpml3d_sweep_inner(const Cpml3dArgs &a, float *s
             for (int m = 1; m <= L; ++m) acc = fmaf(a.wy[m], c[m * PX] + c[-m * PX], acc);
             const float pxy = fmaf(a.w0lo, c[0], acc);  // low part of the centre weight, after cancellation
 
+            float pn = 0.f;
             if (own && live) {
                 float s = 0.f;
                 if (a.nsrc > 0) {
@@ -271,11 +275,13 @@ __device__ __forceinline__ void cpml3d_sweep_inner(const Cpml3dArgs &a, float *s
                     }
                 }
                 const float pc = zq[L];
-                const float pn = pc + fmaf(f_v2[d], (pz + pxy) + s, pc - f_po[d]);
+                pn = pc + fmaf(f_v2[d], (pz + pxy) + s, pc - f_po[d]);
                 st3_nt(pn, rO, vcol, plane(i));
                 if (a.rec && i == 0) a.rec[(size_t)k * nx + j] = pn;
                 if (a.snap) a.snap[((size_t)i * ny + k) * nx + j] = pn;
             }
+            // watchdog on the top (receiver) plane
+            if (a.health && i == 0) report_nonfinite(a.health, own && live && nonfinite(pn), a.step);
             // refill the slot with plane i+PD; past the chunk the loads read the
             // bottom padding (2*PD + L <= kHZbot)
             f_front[d] = ld3(rP, vcol, plane(i + PD + L));
@@ -564,6 +570,7 @@ __device__ __forceinline__ void cpml3d_sweep_strip(const Cpml3dArgs &a, float *s
             pxy = px + py;
             }
 
+            float pn = 0.f;
             if (own && live) {
                 float s = 0.f;
                 if (a.nsrc > 0) {
@@ -575,11 +582,13 @@ __device__ __forceinline__ void cpml3d_sweep_strip(const Cpml3dArgs &a, float *s
                     }
                 }
                 const float pc = zq[L];
-                const float pn = pc + fmaf(f_v2[d], (pz + pxy) + s, pc - f_po[d]);
+                pn = pc + fmaf(f_v2[d], (pz + pxy) + s, pc - f_po[d]);
                 st3_nt(pn, rO, vcol, plane(i));
                 if (a.rec && i == 0) a.rec[(size_t)k * nx + j] = pn;
                 if (a.snap) a.snap[((size_t)i * ny + k) * nx + j] = pn;
             }
+            // watchdog on the top (receiver) plane
+            if (a.health && i == 0) report_nonfinite(a.health, own && live && nonfinite(pn), a.step);
             // next plane's strip values, then the ring refill with plane i+PD
             load_strip(i + 1);
             f_front[d] = ld3(rP, vcol, plane(i + PD + L));
@@ -660,6 +669,7 @@ class Prop3D final : public Prop {
     int ty = 8, zc = 32;
     DevBuf<int> order;  // block launch order for (order_ty, order_zc)
     int order_ty = 0, order_zc = 0;
+    DevBuf<int> health;
 
     Prop3D(const double *v, int64_t nz_, int64_t nx_, int64_t ny_, int N_, int nB_, double dz_,
            double dx_, double dy_, double dt_) {
@@ -671,6 +681,12 @@ class Prop3D final : public Prop {
             fail(SSSI_EINVAL, "3-D grid too large (%lld cells)", (long long)(nz * nx * ny));
         SSSI_HIP(hipGetDevice(&device));
         SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        {
+            double vmax = 0.0;
+            for (int64_t q = 0; q < nz * nx * ny; ++q) vmax = std::max(vmax, v[q]);
+            courant = vmax * dt * std::sqrt(1.0 / (dz * dz) + 1.0 / (dx * dx) + 1.0 / (dy * dy));
+        }
+        health.alloc(1);
         if (const char *e = std::getenv("SSSI_TY3")) {
             const int t = std::atoi(e);
             if (t == 4 || t == 8 || t == 16) ty = t;
@@ -855,9 +871,17 @@ class Prop3D final : public Prop {
         for (auto *b : {&p[0], &p[1]}) SSSI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), stream));
         if (pool.p)
             SSSI_HIP(hipMemsetAsync(pool.p + o_state, 0, (pool.n - o_state) * sizeof(float), stream));
+        arm_health(health.p, stream);
         cur = 0;
         phase = 0;
+        steps_done = 0;
         SSSI_HIP(hipStreamSynchronize(stream));
+    }
+
+    int *health_word() const override { return health.p; }
+
+    void scan_field() override {
+        launch_nonfinite_scan(p[cur].p, p[cur].n, health.p, (int)steps_done, stream);
     }
 
     void set_points(int n, const int *iz, const int *ix, const int *iy, const double *wavelet,
@@ -981,6 +1005,8 @@ class Prop3D final : public Prop {
         }
         if (record) a.rec = rec.p + (size_t)t * nx * ny;
         a.snap = snap;
+        a.health = health.p;
+        a.step = (int)steps_done;
         const int nblk = a.ntx * a.nty * nzc();
         if (order_ty != ty || order_zc != zc) build_order(a.ntx, a.nty, nzc());
         a.order = order.p;
@@ -1008,6 +1034,7 @@ class Prop3D final : public Prop {
             if (sink && sink->dev && t % sink->stride == 0)
                 snap = sink->dev + (size_t)(t / sink->stride - sink->slot0) * cells();
             step(t, record, snap);
+            ++steps_done;
             if (st) st->launches++;
         }
     }

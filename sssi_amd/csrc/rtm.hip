@@ -252,7 +252,23 @@ class Rtm2D {
                 run_single_phase(*s->gV, job_true(*s));
             }
         }
+        SSSI_HIP(hipStreamSynchronize(stream));
+        for (auto *s : todo) check_shot(*s, *s->gV, "V");
         for (auto *s : todo) s->have_true = true;
+    }
+
+    // watchdog of a shot's grid after its phase ran (the stream is idle)
+    void check_shot(const RtmShot &s, const Cpml2dGrid &g, const char *model) {
+        g.scan_nonfinite(0, (int)nt, stream);
+        g.scan_nonfinite(1, (int)nt, stream);
+        const int h = read_health(g.health.p, stream);
+        if (h != kHealthy) {
+            int k = 0;
+            while (k < (int)shots.size() && shots[k].get() != &s) ++k;
+            fail(SSSI_EUNSTABLE,
+                 "shot %d: non-finite wavefield (NaN/Inf) at step %d of the propagation in %s: "
+                 "the time step is likely beyond the scheme's stability limit", k, h + 1, model);
+        }
     }
 
     // the imaging job of every shot: fwd(VS) + checkpoints, rvs(V) + imaging, stack
@@ -287,6 +303,10 @@ class Rtm2D {
             }
         }
         SSSI_HIP(hipStreamSynchronize(stream));
+        for (auto &s : shots) {
+            check_shot(*s, *s->gS, "VS");
+            check_shot(*s, *s->gV, "V");
+        }
     }
 };
 
@@ -329,8 +349,12 @@ int sssi_rtm2d_create(const double *V, const double *VS, int64_t nz, int64_t nx,
                       int ck_format, double eps, sssi_rtm **out) {
     return rtm_guard([&] {
         if (!V || !VS || !out) fail(SSSI_EINVAL, "V, VS and out must not be NULL");
-        validate_common(nz + nBoundary, 2 * (int64_t)nBoundary + 1, 1, nt, nDiffOrder, nBoundary,
+        validate_common(2, nz + nBoundary, 2 * (int64_t)nBoundary + 1, 1, nt, nDiffOrder, nBoundary,
                         dz, dx, 1.0, dt, true);
+        if (nz < 1 || nx < 1 || nz * nx > ((int64_t)1 << 30))
+            fail(SSSI_EINVAL, "model grid %lldx%lld out of range", (long long)nz, (long long)nx);
+        validate_velocity(V, nz * nx, "V");
+        validate_velocity(VS, nz * nx, "VS");
         if (img_stride < 1) fail(SSSI_EINVAL, "img_stride must be >= 1");
         if (ck_format < SSSI_CK_F32 || ck_format > SSSI_CK_BF16)
             fail(SSSI_EINVAL, "unknown checkpoint format %d", ck_format);

@@ -69,8 +69,10 @@ struct ServeJob {
     Job2D job{};
     // results
     std::vector<float> out;
+    int health_h[2] = {kHealthy, kHealthy};  // watchdog words, copied with the result
     hipEvent_t done_ev = nullptr;
     std::string error;
+    int code = SSSI_EDEVICE;  // status of a failed job
 };
 
 class Server {
@@ -206,7 +208,7 @@ class Server {
                     std::lock_guard<std::mutex> lk(mu);
                     used -= drop->bytes;
                     reserved_slots--;
-                    drop->state = kDone;
+                    drop->state = drop->error.empty() ? kDone : kFailed;
                     cv_done.notify_all();
                     cv_helper.notify_all();
                     continue;
@@ -452,6 +454,13 @@ class Server {
             j->out.resize(n);
             SSSI_HIP(hipMemcpyAsync(j->out.data(), src, n * sizeof(float), hipMemcpyDeviceToHost,
                                     stream));
+            // watchdog words of the job's grids (every phase reports into them)
+            const int *hw[2] = {j->kind == kFwd3d ? j->p3->health_word() : j->g->health.p,
+                                j->gs ? j->gs->health.p : nullptr};
+            for (int q = 0; q < 2; ++q)
+                if (hw[q])
+                    SSSI_HIP(hipMemcpyAsync(&j->health_h[q], hw[q], sizeof(int),
+                                            hipMemcpyDeviceToHost, stream));
             SSSI_HIP(hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming));
             SSSI_HIP(hipEventRecord(j->done_ev, stream));
             j->state = kDraining;
@@ -464,6 +473,15 @@ class Server {
             if (j->state == kDraining && hipEventQuery(j->done_ev) == hipSuccess) {
                 std::lock_guard<std::mutex> lk(mu);
                 j->t_done = now_s();  // result is on the host
+                if (j->health_h[0] != kHealthy || j->health_h[1] != kHealthy) {
+                    const int h = std::min(j->health_h[0], j->health_h[1]);
+                    char msg[256];
+                    snprintf(msg, sizeof(msg),
+                             "non-finite wavefield (NaN/Inf) at tick %d: the time step is likely "
+                             "beyond the scheme's stability limit", h);
+                    j->error = msg;
+                    j->code = SSSI_EUNSTABLE;
+                }
                 finished.push_back(j);
                 cv_helper.notify_all();
                 continue;
@@ -493,7 +511,7 @@ class Server {
         auto pred = [&] { return j.state == kDone || j.state == kFailed; };
         if (timeout < 0) cv_done.wait(lk, pred);
         else cv_done.wait_for(lk, std::chrono::duration<double>(timeout), pred);
-        if (j.state == kFailed) fail(SSSI_EDEVICE, "job %lld failed: %s", (long long)id,
+        if (j.state == kFailed) fail(j.code, "job %lld failed: %s", (long long)id,
                                      j.error.c_str());
         return j.state == kDone;
     }
@@ -563,7 +581,8 @@ int sssi_server_submit_fwd2d(sssi_server *s, const double *v, int64_t nz, int64_
                              int src_iz, int src_ix, const double *wavelet, int64_t *job_id) {
     return serve_guard([&] {
         if (!v || !wavelet) fail(SSSI_EINVAL, "v and wavelet must not be NULL");
-        validate_common(nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_common(2, nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_velocity(v, nz * nx, "v");
         check_src(nz, src_iz, "z");
         check_src(nx, src_ix, "x");
         auto j = std::make_unique<ServeJob>();
@@ -583,7 +602,9 @@ int sssi_server_submit_rtm2d(sssi_server *s, const double *V, const double *VS, 
                              int img_stride, int ck_format, double eps, int64_t *job_id) {
     return serve_guard([&] {
         if (!V || !VS || !wavelet) fail(SSSI_EINVAL, "V, VS and wavelet must not be NULL");
-        validate_common(nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_common(2, nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
+        validate_velocity(V, nz * nx, "V");
+        validate_velocity(VS, nz * nx, "VS");
         check_src(nx, src_ix, "x");
         if (img_stride < 1) fail(SSSI_EINVAL, "img_stride must be >= 1");
         if (ck_format < SSSI_CK_F32 || ck_format > SSSI_CK_BF16)
@@ -607,7 +628,8 @@ int sssi_server_submit_fwd3d(sssi_server *s, const double *v, int64_t nz, int64_
                              const double *wavelet, int64_t *job_id) {
     return serve_guard([&] {
         if (!v || !wavelet) fail(SSSI_EINVAL, "v and wavelet must not be NULL");
-        validate_common(nz, nx, ny, nt, nDiffOrder, nBoundary, dz, dx, dy, dt, true);
+        validate_common(3, nz, nx, ny, nt, nDiffOrder, nBoundary, dz, dx, dy, dt, true);
+        validate_velocity(v, nz * nx * ny, "v");
         check_src(nz, src_iz, "z");
         check_src(nx, src_ix, "x");
         check_src(ny, src_iy, "y");
@@ -639,6 +661,8 @@ int sssi_server_result(sssi_server *s, int64_t job_id, double *out, int64_t n) {
         Server &sv = S(s);
         std::lock_guard<std::mutex> lk(sv.mu);
         ServeJob &j = sv.get(job_id);
+        if (j.state == kFailed)
+            fail(j.code, "job %lld failed: %s", (long long)job_id, j.error.c_str());
         if (j.state != kDone) fail(SSSI_ESTATE, "job %lld not finished", (long long)job_id);
         if (!out || n != (int64_t)j.out.size())
             fail(SSSI_EINVAL, "result of job %lld has %zu values", (long long)job_id,

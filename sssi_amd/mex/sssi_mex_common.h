@@ -35,8 +35,12 @@ static double sssi_mex_scalar(const mxArray *a, const char *name)
     return mxGetScalar(a);
 }
 
+/* Library status -> MATLAB error: a non-finite wavefield (the watchdog's
+ * SSSI_EUNSTABLE, a time step beyond the stability limit) gets its own
+ * identifier; every other failure is sssi:run. */
 static void sssi_mex_fail(int rc)
 {
+    if (rc == SSSI_EUNSTABLE) mexErrMsgIdAndTxt("sssi:unstable", "%s", sssi_last_error());
     if (rc != SSSI_OK) mexErrMsgIdAndTxt("sssi:run", "%s", sssi_last_error());
 }
 

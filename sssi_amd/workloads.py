@@ -1,0 +1,120 @@
+"""Seeded synthetic workloads of the five benchmark configurations.
+
+Each builder returns plain numpy inputs (models indexed [iz, ix(, iy)], Ricker
+wavelets, source positions) so that the same inputs drive the GPU path, the
+tests and the benchmark.
+
+  config1()      2-D CPML forward, 100 x 200 + nB 20 (120 x 240), N 2, nt 1000,
+                 dt 1 ms, 20 Hz Ricker at (0, 100 + nB), every-10th snapshots
+  config2()      2-D CPML forward, 1000 x 3000 + nB 40 (1040 x 3080), N 5,
+                 nt 4000, dt 0.5 ms, 25 Hz Ricker at the surface centre
+  config3(n)     2-D RTM, n shots (64) of width U[400, 1200] on a 600 x 6000
+                 layered model and its 10-cell Gaussian blur, N 4, nB 30,
+                 nt 3000, dt 1 ms, 15 Hz, imaging every 5 steps
+  config4()      3-D CPML forward, 400^3 + nB 20 (420 x 440 x 440), N 4,
+                 nt 2000, dt 1 ms, 15 Hz Ricker at the surface centre
+  config5(n)     serving mix: 70 % 2-D forward / RTM-shot jobs (200..2000 x
+                 200..4000, N 2..5, nt 500..3000), 30 % 3-D forward (128^3 ..
+                 256^3, nt 500..1500); layered 1500-4500 m/s; 10-30 Hz Ricker
+"""
+import numpy as np
+
+from . import synth
+
+
+def config1():
+    nz0, nx0, nB, N = 100, 200, 20, 2
+    d, dt, nt = 10.0, 1e-3, 1000
+    v = synth.layered_2d(nz0, nx0, 5, 1500.0, 4500.0, seed=1)
+    V = synth.extBoundary(v, nB, 2)
+    w = synth.ricker(20.0, nt, dt)
+    return dict(V=V, nB=nB, N=N, dz=d, dx=d, dt=dt, nt=nt, wavelet=w, src_iz=[0],
+                src_ix=[nx0 // 2 + nB], snap_stride=10)
+
+
+def config2():
+    nz0, nx0, nB, N = 1000, 3000, 40, 5
+    dz = dx = 5.0
+    dt, nt = 5e-4, 4000
+    v = synth.layered_2d(nz0, nx0, 20, 1500.0, 5000.0, seed=2)
+    v = v * (1.0 + synth.gaussian_perturbation(v.shape, 0.05, 10.0, seed=3))
+    v = np.clip(v, 1500.0, 5000.0)
+    V = synth.extBoundary(v, nB, 2)
+    w = synth.ricker(25.0, nt, dt)
+    return dict(V=V, nB=nB, N=N, dz=dz, dx=dx, dt=dt, nt=nt, wavelet=w, src_iz=[0],
+                src_ix=[nx0 // 2 + nB])
+
+
+def config3(nshots=64, seed=5):
+    nz, nx, nB, N = 600, 6000, 30, 4
+    d, dt, nt = 10.0, 1e-3, 3000
+    V = synth.layered_2d(nz, nx, 12, 1500.0, 4500.0, seed=seed)
+    VS = synth.smooth_copy(V, 10)
+    rng = np.random.default_rng(seed + 1)
+    w = synth.ricker(15.0, nt, dt)
+    shots = []
+    for _ in range(nshots):
+        xc = int(rng.integers(400, 1201))
+        xb = int(rng.integers(0, nx - xc + 1))
+        shots.append((xb, xc, xc // 2))
+    return dict(V=V, VS=VS, nB=nB, N=N, d=d, dt=dt, nt=nt, wavelet=w, shots=shots,
+                img_stride=5)
+
+
+def config4():
+    n0, nB, N = 400, 20, 4
+    d, dt, nt = 10.0, 1e-3, 2000
+    v = synth.cake_3d_dipping(n0, n0, n0, 8, 1500.0, 4500.0, seed=4)
+    # the 8th-order 3-D scheme at dt = 1 ms, d = 10 m is stable below 4490 m/s
+    v = np.clip(v, 1500.0, 4450.0)
+    V = synth.extBoundary(v, nB, 3)
+    w = synth.ricker(15.0, nt, dt)
+    c = (n0 + 2 * nB) // 2
+    return dict(V=V, nB=nB, N=N, d=d, dt=dt, nt=nt, wavelet=w, src=(0, c, c))
+
+
+def config5(njobs=60, seed=7):
+    """Job list [(kind, kwargs)] for sssi_amd.Server.submit_<kind>.  RTM-shot
+    jobs carry an extBoundary-extended model (nB = 20) so that each one is also
+    a one-shot Rtm2D migration of the physical model over its whole width."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(njobs):
+        f = float(rng.uniform(10, 30))
+        if rng.random() < 0.7:
+            nz, nx = int(rng.integers(200, 2001)), int(rng.integers(200, 4001))
+            N = int(rng.integers(2, 6))
+            nt = int(rng.integers(500, 3001))
+            v = synth.layered_2d(nz, nx, 6, 1500.0, 4500.0, seed=seed * 1000 + k)
+            nB = 20
+            dt = 0.5 * 10.0 / (4500.0 * np.sqrt(2) * float(np.sum(np.abs(synth.dCoef(N)))))
+            w = synth.ricker(f, nt, dt)
+            if rng.random() < 0.5:
+                jobs.append(("fwd2d", dict(v=v, nt=nt, nDiffOrder=N, nBoundary=nB, dz=10.0,
+                                           dx=10.0, dt=dt, src_iz=0, src_ix=nx // 2, wavelet=w)))
+            else:
+                pz, px = nz - nB, nx - 2 * nB
+                vp = v[:pz, :px]
+                jobs.append(("rtm2d", dict(V=synth.extBoundary(vp, nB, 2),
+                                           VS=synth.extBoundary(synth.smooth_copy(vp, 5), nB, 2),
+                                           nt=nt, nDiffOrder=N, nBoundary=nB, dz=10.0, dx=10.0,
+                                           dt=dt, src_ix=nB + px // 2, wavelet=w, img_stride=5,
+                                           ck_format="bf16")))
+        else:
+            n = int(rng.integers(128, 257))
+            nt = int(rng.integers(500, 1501))
+            N = int(rng.integers(2, 6))
+            v = synth.cake_3d_dipping(n, n, n, 4, 1500.0, 4500.0, seed=seed * 1000 + k)
+            dt = 0.5 * 10.0 / (4500.0 * np.sqrt(3) * float(np.sum(np.abs(synth.dCoef(N)))))
+            jobs.append(("fwd3d", dict(v=v, nt=nt, nDiffOrder=N, nBoundary=16, dz=10.0, dx=10.0,
+                                       dy=10.0, dt=dt, src=(0, n // 2, n // 2),
+                                       wavelet=synth.ricker(f, nt, dt))))
+    return jobs
+
+
+def job_cells(kind, kw):
+    """Cell updates of a config5 job (RTM shots propagate three times)."""
+    if kind == "fwd3d":
+        return int(np.prod(kw["v"].shape)) * kw["nt"]
+    v = kw["v"] if kind == "fwd2d" else kw["V"]
+    return int(np.prod(v.shape)) * kw["nt"] * (3 if kind == "rtm2d" else 1)

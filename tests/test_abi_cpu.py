@@ -110,3 +110,85 @@ def test_cfl_number():
     # config with v dt/dx = 0.45 and N = 2 in 2-D: 0.45 * sqrt(2) * (9/8 + 1/24)
     c = synth.cfl_number(4500.0, 1e-3, [10.0, 10.0], 2)
     assert c == pytest.approx(0.45 * np.sqrt(2) * (9 / 8 + 1 / 24))
+
+
+def _rc(name, *args):
+    lib = _lib.lib()
+    rc = getattr(lib, name)(*args)
+    return rc, lib.sssi_last_error().decode()
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, 0.0, -1500.0])
+def test_velocity_validated_before_device_work(bad):
+    """A NaN/Inf or non-positive velocity is rejected (SSSI_EINVAL) by every
+    entry point that takes a model, before any device call (none runs here)."""
+    from sssi_amd._lib import _dp, _f64
+    import ctypes
+    nz, nx, nt = 20, 30, 8
+    v = np.full((nz, nx), 2000.0)
+    v[7, 11] = bad
+    vf = _f64(v)
+    w = np.ones(nt)
+    iz = np.zeros(1, np.int32)
+    ix = np.full(1, 5, np.int32)
+    I = ctypes.POINTER(ctypes.c_int)
+    data = np.zeros((nx, nt), order="F")
+    calls = [
+        ("sssi_fwd_cpml_2d_points", _dp(vf), nz, nx, nt, 1, iz.ctypes.data_as(I),
+         ix.ctypes.data_as(I), _dp(w), 2, 4, 10.0, 10.0, 1e-3, _dp(data), None, 1),
+        ("sssi_fwd_sponge_2d_points", _dp(vf), nz, nx, nt, 1, iz.ctypes.data_as(I),
+         ix.ctypes.data_as(I), _dp(w), 4, 10.0, 10.0, 1e-3, _dp(data), None, 1),
+        ("sssi_rvs_cpml_2d_ex", _dp(vf), nz, nx, _dp(_f64(data)), nt, 2, 4, 10.0, 10.0, 1e-3,
+         None, None, 1),
+        ("sssi_prop2d_create", _dp(vf), nz, nx, 0, 2, 4, 10.0, 10.0, 1e-3,
+         ctypes.byref(ctypes.c_void_p())),
+        ("sssi_rtm2d_create", _dp(vf), _dp(vf), nz, nx, 2, 4, 10.0, 10.0, 1e-3, nt, 1, 1, 1e-3,
+         ctypes.byref(ctypes.c_void_p())),
+    ]
+    for name, *args in calls:
+        rc, msg = _rc(name, *args)
+        assert rc == -1, (name, rc, msg)
+        assert "finite and positive" in msg and "element 227" in msg  # column-major: 11 * 20 + 7, (name, msg)
+    v3 = np.full((6, 8, 10), 2000.0)
+    v3[1, 2, 3] = bad
+    rc, msg = _rc("sssi_prop3d_create", _dp(_f64(v3)), 6, 8, 10, 2, 2, 10.0, 10.0, 10.0, 1e-3,
+                  ctypes.byref(ctypes.c_void_p()))
+    assert rc == -1 and "finite and positive" in msg
+
+
+def test_3d_rejects_boundary_wider_than_a_single_y_slice():
+    """A 3-D call with ny = 1 (a 2-D model) and nBoundary >= 1 has no room for
+    the two y strips; the reference errors on it, and so does every 3-D entry."""
+    import ctypes
+    from sssi_amd._lib import _dp, _f64
+    v = _f64(np.full((12, 16, 1), 2000.0))
+    rc, msg = _rc("sssi_prop3d_create", _dp(v), 12, 16, 1, 2, 2, 10.0, 10.0, 10.0, 1e-3,
+                  ctypes.byref(ctypes.c_void_p()))
+    assert rc == -1 and "ny" in msg
+    with pytest.raises(sssi_amd.SssiError) as e:
+        sssi_amd.fwd_cpml_3d_points(np.full((12, 16, 1), 2000.0), 4, 2, 1, 10.0, 10.0, 10.0, 1e-3,
+                                    [0], [8], [0], np.ones(4))
+    assert e.value.code == -1 and "ny" in str(e.value)
+
+
+def test_sponge_grid_capped_below_2gib():
+    """The sponge kernel drops out-of-model stores with a 2^31 byte offset, which
+    is out of range only below 2 GiB: larger padded grids are refused before
+    anything is read or allocated."""
+    import ctypes
+    from sssi_amd._lib import _dp
+    v = np.full(4, 2000.0)  # never read: the size check comes first
+    nz, nx = 16384, 32704
+    rc, msg = _rc("sssi_prop2d_create", _dp(v), nz, nx, 1, 1, 4, 10.0, 10.0, 1e-3,
+                  ctypes.byref(ctypes.c_void_p()))
+    assert rc == -1 and "2 GiB" in msg
+    rc, msg = _rc("sssi_fwd_sponge_2d_points", _dp(v), nz, nx, 4, 0, None, None, None, 4, 10.0,
+                  10.0, 1e-3, None, None, 1)
+    assert rc == -1 and "2 GiB" in msg
+
+
+def test_unstable_status_code_is_exported():
+    text = open(os.path.join(ROOT, "include", "sssi.h")).read()
+    assert re.search(r"#define SSSI_EUNSTABLE \(-6\)", text)
+    src = open(os.path.join(ROOT, "sssi_amd", "csrc", "prop2d.hip")).read()
+    assert "SSSI_EUNSTABLE" in src  # raised by the watchdog check
