@@ -46,41 +46,36 @@ void staggered_coefficients(int N, double *c) {
 
 // fwdTimeCpmlFor2dAw_mex.c:100-145 with dampPml (finiteDifference.c:253-280):
 // d0 = -(3 v)/(2 L) log(1e-6), d = d0 (u/L)(u/L), b = exp(-d dt).
+static inline float bm1_of(double v, double u, double L, double dt) {
+    const double d0 = -(3.0 * v) / (2 * L) * std::log(1e-6);
+    const double d = d0 * (u / L) * (u / L);
+    return (float)(std::exp(-d * dt) - 1.0);
+}
+
+float cpml2d_bm1_x(const double *v, int64_t nz, int64_t nx, int nB, double dx, double dt,
+                   int64_t q) {
+    const int64_t i = q / (2 * nB), xs = q % (2 * nB);
+    const double L = nB * dx;
+    if (xs < nB) return bm1_of(v[xs * nz + i], (nB - xs) * dx, L, dt);         // left
+    const int64_t j = xs - nB;
+    return bm1_of(v[(nx - nB + j) * nz + i], (j + 1) * dx, L, dt);              // right
+}
+
+float cpml2d_bm1_z(const double *v, int64_t nz, int64_t nx, int nB, double dz, double dt,
+                   int64_t q) {
+    const int64_t i = q / nx, j = q % nx;
+    return bm1_of(v[j * nz + (nz - nB + i)], (i + 1) * dz, nB * dz, dt);      // bottom
+}
+
 void cpml2d_bm1(const double *v, int64_t nz, int64_t nx, int nB, double dz, double dx, double dt,
                 std::vector<float> &zbm1, std::vector<float> &xbm1) {
-    const double logR = std::log(1e-6);
     zbm1.assign((size_t)nB * nx, 0.f);
     xbm1.assign((size_t)nz * 2 * nB, 0.f);
     if (nB == 0) return;
-    {
-        const double L = nB * dx;
-        for (int j = 0; j < nB; ++j)
-            for (int64_t i = 0; i < nz; ++i) {
-                const double u = (nB - j) * dx;
-                const double d0 = -(3.0 * v[(int64_t)j * nz + i]) / (2 * L) * logR;
-                const double d = d0 * (u / L) * (u / L);
-                xbm1[(size_t)i * 2 * nB + j] = (float)(std::exp(-d * dt) - 1.0);
-            }
-        for (int j = 0; j < nB; ++j)
-            for (int64_t i = 0; i < nz; ++i) {
-                const int64_t jj = nx - nB + j;
-                const double u = (j + 1) * dx;
-                const double d0 = -(3.0 * v[jj * nz + i]) / (2 * L) * logR;
-                const double d = d0 * (u / L) * (u / L);
-                xbm1[(size_t)i * 2 * nB + nB + j] = (float)(std::exp(-d * dt) - 1.0);
-            }
-    }
-    {
-        const double L = nB * dz;
-        for (int64_t j = 0; j < nx; ++j)
-            for (int i = 0; i < nB; ++i) {
-                const int64_t ii = nz - nB + i;
-                const double u = (i + 1) * dz;
-                const double d0 = -(3.0 * v[j * nz + ii]) / (2 * L) * logR;
-                const double d = d0 * (u / L) * (u / L);
-                zbm1[(size_t)i * nx + j] = (float)(std::exp(-d * dt) - 1.0);
-            }
-    }
+    for (int64_t q = 0; q < (int64_t)xbm1.size(); ++q)
+        xbm1[q] = cpml2d_bm1_x(v, nz, nx, nB, dx, dt, q);
+    for (int64_t q = 0; q < (int64_t)zbm1.size(); ++q)
+        zbm1[q] = cpml2d_bm1_z(v, nz, nx, nB, dz, dt, q);
 }
 
 void validate_common(int dim, int64_t nz, int64_t nx, int64_t ny, int64_t nt, int N, int nB,

@@ -126,6 +126,11 @@ void staggered_coefficients(int N, double *c);
 //   xbm1[i*2nB + xs(j)]        columns 0..nB-1 and nx-nB..nx-1, all rows
 void cpml2d_bm1(const double *v, int64_t nz, int64_t nx, int nB, double dz, double dx, double dt,
                 std::vector<float> &zbm1, std::vector<float> &xbm1);
+// Element q of those tables (q < nz*2nB for x, q < nB*nx for z).
+float cpml2d_bm1_x(const double *v, int64_t nz, int64_t nx, int nB, double dx, double dt,
+                   int64_t q);
+float cpml2d_bm1_z(const double *v, int64_t nz, int64_t nx, int nB, double dz, double dt,
+                   int64_t q);
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 

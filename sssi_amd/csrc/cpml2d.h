@@ -63,6 +63,11 @@ struct Job2D {
     int *health;
 };
 
+// Stencil weights of a 2-D CPML job (Job2D fields cz .. w0xlo): staggered
+// coefficients c_k / d for the two-stage form and the composite D-D+ weights
+// / d^2, each centre split as w + wlo so that the FP32 weights sum to zero.
+void stencil_2d(int N, double dz, double dx, Job2D &J);
+
 // Per-model device state of a 2-D CPML propagation (velocity-dependent
 // coefficients and damping, wavefield levels, memory variables).
 class Cpml2dGrid {
@@ -76,8 +81,7 @@ class Cpml2dGrid {
     DevBuf<float> p[2], vdt2;
     DevBuf<float> zphi[2], zpsi, zbm1, xphi[2], xpsi, xbm1;
     DevBuf<int> health;  // non-finite watchdog word of the jobs on this grid
-    float cz[8] = {0}, cx[8] = {0}, w0 = 0.f, w0lo = 0.f, wz[16] = {0}, wx[16] = {0};
-    float w0z = 0.f, w0zlo = 0.f, w0x = 0.f, w0xlo = 0.f;
+    Job2D coef{};  // stencil weights (stencil_2d); job() copies them
 
     // rz 0 / ng -1: the SSSI_RZ / SSSI_NG environment variables or the defaults
     Cpml2dGrid(const double *v, int64_t nz, int64_t nx, int N, int nB, double dz, double dx,
@@ -131,6 +135,8 @@ void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector
 // Launch one step of a single job (descriptor by value).  With `base`, the
 // kernel's step is step + *base (a launch inside a captured time loop).
 void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
+                          int step, hipStream_t s, const int *base = nullptr);
+void launch_cpml2d_single(int N, int rz, int ng, const Job2D &job, const int2 *order, int nblk,
                           int step, hipStream_t s, const int *base = nullptr);
 // Launch one step of many jobs sharing nDiffOrder, rz and march layout
 // (descriptors in device memory).

@@ -652,6 +652,39 @@ static void composite_weights(int N, const double *c, double *w) {
         }
 }
 
+void stencil_2d(int N, double dz, double dx, Job2D &J) {
+    double c[8];
+    staggered_coefficients(N, c);
+    for (int k = 0; k < 8; ++k) J.cz[k] = J.cx[k] = 0.f;
+    for (int m = 0; m < 16; ++m) J.wz[m] = J.wx[m] = 0.f;
+    for (int k = 0; k < N; ++k) {
+        J.cz[k] = (float)(c[k] / dz);
+        J.cx[k] = (float)(c[k] / dx);
+    }
+    double w[32];
+    composite_weights(N, c, w);
+    const int L = 2 * N - 1;
+    double side = 0.0, sidez = 0.0, sidex = 0.0;  // exact in double: sums of floats
+    for (int m = 1; m <= L; ++m) {
+        J.wz[m] = (float)(w[L + m] / (dz * dz));
+        J.wx[m] = (float)(w[L + m] / (dx * dx));
+        side += 2.0 * ((double)J.wz[m] + (double)J.wx[m]);
+        sidez += 2.0 * (double)J.wz[m];
+        sidex += 2.0 * (double)J.wx[m];
+    }
+    // The centre weight is chosen so that the FP32 weights sum to zero (the
+    // Laplacian of a constant vanishes, as in the two-stage form where
+    // D+ of a constant is exactly 0); it is carried as w0 + w0lo.  The
+    // per-direction centres serve strip tiles that use the composite stencil
+    // in one direction only.
+    J.w0 = (float)(-side);
+    J.w0lo = (float)(-side - (double)J.w0);
+    J.w0z = (float)(-sidez);
+    J.w0zlo = (float)(-sidez - (double)J.w0z);
+    J.w0x = (float)(-sidex);
+    J.w0xlo = (float)(-sidex - (double)J.w0x);
+}
+
 Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB_, double dz_,
                        double dx_, double dt_, int rz_, int ng_)
     : nz(nz_), nx(nx_), N(N_), nB(nB_), dz(dz_), dx(dx_), dt(dt_) {
@@ -693,34 +726,7 @@ Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB
         }, 16);
         SSSI_HIP(hipMemcpy(vdt2.p, hv.p, np * sizeof(float), hipMemcpyHostToDevice));
     }
-    double c[8];
-    staggered_coefficients(N, c);
-    for (int k = 0; k < N; ++k) {
-        cz[k] = (float)(c[k] / dz);
-        cx[k] = (float)(c[k] / dx);
-    }
-    double w[32];
-    composite_weights(N, c, w);
-    const int L = 2 * N - 1;
-    double side = 0.0, sidez = 0.0, sidex = 0.0;  // exact in double: sums of floats
-    for (int m = 1; m <= L; ++m) {
-        wz[m] = (float)(w[L + m] / (dz * dz));
-        wx[m] = (float)(w[L + m] / (dx * dx));
-        side += 2.0 * ((double)wz[m] + (double)wx[m]);
-        sidez += 2.0 * (double)wz[m];
-        sidex += 2.0 * (double)wx[m];
-    }
-    // The centre weight is chosen so that the FP32 weights sum to zero (the
-    // Laplacian of a constant vanishes, as in the two-stage form where
-    // D+ of a constant is exactly 0); it is carried as w0 + w0lo.  The
-    // per-direction centres serve strip tiles that use the composite stencil
-    // in one direction only.
-    w0 = (float)(-side);
-    w0lo = (float)(-side - (double)w0);
-    w0z = (float)(-sidez);
-    w0zlo = (float)(-sidez - (double)w0z);
-    w0x = (float)(-sidex);
-    w0xlo = (float)(-sidex - (double)w0x);
+    stencil_2d(N, dz, dx, coef);
     if (nB > 0) {
         std::vector<float> zb, xb;
         cpml2d_bm1(v, nz, nx, nB, dz, dx, dt, zb, xb);
@@ -840,14 +846,14 @@ Job2D Cpml2dGrid::job() const {
     J.nz = (int)nz; J.nx = (int)nx; J.pitch = (int)pitch; J.rows = (int)rows;
     J.ntx = ntx(); J.nwz = nwz(); J.nB = nB;
     J.mz_rows = mz_rows; J.mta = mta; J.mntx = mntx;
-    for (int k = 0; k < 8; ++k) { J.cz[k] = cz[k]; J.cx[k] = cx[k]; }
-    J.w0 = w0;
-    J.w0lo = w0lo;
-    J.w0z = w0z;
-    J.w0zlo = w0zlo;
-    J.w0x = w0x;
-    J.w0xlo = w0xlo;
-    for (int m = 0; m < 16; ++m) { J.wz[m] = wz[m]; J.wx[m] = wx[m]; }
+    for (int k = 0; k < 8; ++k) { J.cz[k] = coef.cz[k]; J.cx[k] = coef.cx[k]; }
+    J.w0 = coef.w0;
+    J.w0lo = coef.w0lo;
+    J.w0z = coef.w0z;
+    J.w0zlo = coef.w0zlo;
+    J.w0x = coef.w0x;
+    J.w0xlo = coef.w0xlo;
+    for (int m = 0; m < 16; ++m) { J.wz[m] = coef.wz[m]; J.wx[m] = coef.wx[m]; }
     J.p[0] = p[0].p; J.p[1] = p[1].p;
     J.vdt2 = vdt2.p;
     J.zphi[0] = zphi[0].p; J.zphi[1] = zphi[1].p; J.zpsi = zpsi.p; J.zbm1 = zbm1.p;
@@ -966,14 +972,19 @@ void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector
                       (RZ), (NG));                                                        \
     }
 
-void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
+void launch_cpml2d_single(int N, int rz, int ng, const Job2D &job, const int2 *order, int nblk,
                           int step, hipStream_t s, const int *base) {
 #define SSSI_L1(NN, R, G)                                                                 \
     hipLaunchKernelGGL((cpml2d_single<NN, R, G>), dim3(nblk), dim3(256), 0, s, job, order, step, \
                        base)
-    SSSI_NRZ_SWITCH(g.N, g.rz, g.ng, SSSI_L1)
+    SSSI_NRZ_SWITCH(N, rz, ng, SSSI_L1)
 #undef SSSI_L1
     SSSI_HIP(hipGetLastError());
+}
+
+void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
+                          int step, hipStream_t s, const int *base) {
+    launch_cpml2d_single(g.N, g.rz, g.ng, job, order, nblk, step, s, base);
 }
 
 void launch_cpml2d_batch(int N, int rz, int ng, const Job2D *jobs, const int2 *order, int nblk,
