@@ -115,6 +115,21 @@ int sssi_fwd_cpml_2d_points(const double *v, int64_t nz, int64_t nx, int64_t nt,
                             int nDiffOrder, int nBoundary, double dz, double dx, double dt,
                             double *data, double *snapshot, int64_t snap_stride);
 
+/* No counterpart in the reference: sssi_fwd_cpml_2d_points for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results, bit
+ * for bit; every device buffer lives in one library-held array of 64 MiB per
+ * GPU (allocated with the library's code, kept until the process ends), and
+ * the call makes no device or host allocation.  Problems that need more than
+ * the array (roughly nz*nx*12 B for the levels plus nt*nx*4 B of receivers
+ * plus 2 MiB of staging), or more than 64 sources, are refused (SSSI_EINVAL).
+ * Calls on one GPU share its array and are serialised by the library. */
+int sssi_fwd_cpml_2d_points_noalloc(const double *v, int64_t nz, int64_t nx, int64_t nt,
+                                    int nsrc, const int *src_iz, const int *src_ix,
+                                    const double *wavelet, int nDiffOrder, int nBoundary,
+                                    double dz, double dx, double dt, double *data,
+                                    double *snapshot, int64_t snap_stride);
+
 int sssi_rvs_cpml_2d_ex(const double *v, int64_t nz, int64_t nx, const double *data, int64_t nt,
                         int nDiffOrder, int nBoundary, double dz, double dx, double dt,
                         double *model, double *snapshot, int64_t snap_stride);

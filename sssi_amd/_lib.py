@@ -33,6 +33,8 @@ SIGNATURES = {
     "sssi_rvsTimeSpongeFor2dAw": [_D, _i64, _i64, _D, _i64, _int, _dbl, _dbl, _dbl, _D, _D],
     "sssi_fwd_cpml_2d_points": [_D, _i64, _i64, _i64, _int, _I, _I, _D, _int, _int, _dbl, _dbl,
                                 _dbl, _D, _D, _i64],
+    "sssi_fwd_cpml_2d_points_noalloc": [_D, _i64, _i64, _i64, _int, _I, _I, _D, _int, _int, _dbl,
+                                        _dbl, _dbl, _D, _D, _i64],
     "sssi_rvs_cpml_2d_ex": [_D, _i64, _i64, _D, _i64, _int, _int, _dbl, _dbl, _dbl, _D, _D, _i64],
     "sssi_fwd_cpml_3d_points": [_D, _i64, _i64, _i64, _i64, _int, _I, _I, _I, _D, _int, _int,
                                 _dbl, _dbl, _dbl, _dbl, _D, _D, _i64],
