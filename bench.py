@@ -1,20 +1,27 @@
-"""Benchmark: 2-D acoustic CPML forward modelling, single shot, on one MI355X.
+"""Benchmark: the acoustic CPML propagators and RTM on one MI355X.
 
-Workload (FP32 on the GPU): 1000 x 3000 physical grid, dz = dx = 5 m, seeded
-20-layer model (1500-5000 m/s) plus a smooth Gaussian random perturbation
-(sigma 5 %, 50 m correlation), clipped to 1500-5000 m/s; nBoundary = 40 CPML
-cells (extended grid 1040 x 3080); nDiffOrder = 5 (10th order in space);
-dt = 0.5 ms; nt = 4000; 25 Hz Ricker at the surface centre; receivers at all
-3080 surface columns.
-
-One benchmark "step" is one whole propagation (4000 time steps) of that shot
-with the model, damping profiles, source and wavefields already resident in
-HBM.  `value` is Gcell-updates/s over the K timed steps.
-
-Prints one JSON line with the metric, a roofline object for the time-step
-kernel (algorithmic bytes per launch / average launch duration measured with
-HIP events on the kernel's stream) and a CPU baseline (the FP64 oracle on a
+Headline (the default line's top-level fields): 2-D CPML forward modelling,
+single shot.  1000 x 3000 physical grid, dz = dx = 5 m, seeded 20-layer model
+(1500-5000 m/s) plus a smooth Gaussian random perturbation (sigma 5 %, 50 m
+correlation), clipped to 1500-5000 m/s; nBoundary = 40 CPML cells (extended
+grid 1040 x 3080); nDiffOrder = 5 (10th order in space); dt = 0.5 ms;
+nt = 4000; 25 Hz Ricker at the surface centre; receivers at all 3080 surface
+columns.  One benchmark "step" is one whole propagation (4000 time steps) of
+that shot with the model, damping profiles, source and wavefields already
+resident in HBM.  `value` is Gcell-updates/s over the K timed steps.  The line
+carries a roofline object for the time-step kernel (algorithmic bytes per
+launch / average launch duration measured with HIP events on the kernel's
+stream) and a CPU baseline (the FP64 oracle, one shot per host core, on a
 bounded sample of the same workload).
+
+Sub-objects of the same line, each with its own fixed step counts:
+  cfg4_3d     3-D CPML forward, 420 x 440 x 440, nDiffOrder 4, nt 2000
+              (2 timed passes, roofline from its own event timing)
+  cfg3_rtm    64-shot RTM: plain FP32 one shot per launch, batched FP32,
+              batched 16-bit checkpoints, served (shots/hour)
+  cfg5_serve  the 60-job Poisson serving mix: p50/p99 latency and throughput,
+              continuous batching vs one job at a time
+`--workload` runs one of them alone (and the sponge / copy calibrations).
 """
 import argparse
 import json
@@ -46,18 +53,43 @@ def traffic_from_profiles(name="cpml2d_step_bench_traffic.json"):
         return None
 
 
-def cpu_baseline(wl, steps=60):
+def cpu_baseline(wl, steps=None, cores=None):
+    """The FP64 oracle on the host cores, one shot per core: each core runs its
+    own copy of the workload's shot single-threaded for the first `steps` time
+    steps (about 10 s of CPU work), all cores at once; the value is the
+    aggregate cell-update rate."""
+    import threading
     import oracle
-    threads = oracle.num_threads()
+    cores = cores or min(16, os.cpu_count() or 1)
+    nz, nx = wl["V"].shape
+    if steps is None:
+        steps = max(5, int(10.0 * 2e7 / (nz * nx)))  # ~10 s at ~0.02 Gcell/s per core
+    steps = min(steps, wl["nt"])
+    errors = []
+
+    def shot():
+        try:
+            oracle.set_threads(1)
+            oracle.fwd_cpml_2d(wl["V"], steps, wl["N"], wl["nB"], wl["dz"], wl["dx"], wl["dt"],
+                               src_iz=wl["src_iz"], src_ix=wl["src_ix"],
+                               wavelet=wl["wavelet"][:steps])
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=shot) for _ in range(cores)]
     t0 = time.perf_counter()
-    oracle.fwd_cpml_2d(wl["V"], steps, wl["N"], wl["nB"], wl["dz"], wl["dx"], wl["dt"],
-                       src_iz=wl["src_iz"], src_ix=wl["src_ix"], wavelet=wl["wavelet"][:steps])
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
     el = time.perf_counter() - t0
-    cells = wl["V"].size
-    return {"value": round(cells * steps / el / 1e9, 4), "unit": "Gcell-updates/s",
-            "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, same {wl['V'].shape[0]}x{wl['V'].shape[1]} grid and shot, "
-                      f"first {steps} of {wl['nt']} time steps, {threads} OpenMP threads"}
+    if errors:
+        raise errors[0]
+    return {"value": round(cores * nz * nx * steps / el / 1e9, 4), "unit": "Gcell-updates/s",
+            "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, one shot per core on {cores} host cores at once, each the "
+                      f"same {nz}x{nx} grid and shot for the first {steps} of {wl['nt']} time "
+                      f"steps, single-threaded ({el:.1f} s)"}
 
 
 def workload_3d():
@@ -89,6 +121,7 @@ def bench_3d(args):
         kern_ms += ms
         launches += n
     wall = time.perf_counter() - t0
+    P.close()
     avg = kern_ms / 1e3 / launches
     ach = bytes_step / avg / 1e9
     return {
@@ -157,26 +190,61 @@ def workload_rtm(nshots=64, seed=5):
 
 
 def bench_rtm(args):
-    """Config-3 RTM: 64 shots of uneven aperture; plain (one shot per launch, FP32
-    checkpoints) vs batched (all shots per launch, 16-bit scaled checkpoints)."""
+    """Config-3 RTM: 64 shots of uneven aperture on the 600 x 6000 model.
+
+    Modes: plain (one shot per launch, FP32 checkpoints); batched (one launch per
+    time step covers every shot) with FP32 and with 16-bit scaled checkpoints,
+    which separates the two effects; served (every shot a job of the serving
+    loop, 16-bit checkpoints).  dataTrue = fwd(V) is cached per shot as the
+    reference's apps cache it (mainRtmTimeCpmlFor2dAw.m:170-184); shots/hour is
+    given without it and, for the plain and served modes, with it (a served
+    job computes its own dataTrue)."""
     import sssi_amd
+    from sssi_amd import synth
     wl = workload_rtm(args.shots)
+    nshot = len(wl["shots"])
     res = {}
-    for name, batched, fmt in (("plain_fp32", False, "fp32"), ("batched_" + args.ck, True, args.ck)):
+    for name, batched, fmt in (("plain_fp32", False, "fp32"), ("batched_fp32", True, "fp32"),
+                               ("batched_" + args.ck, True, args.ck)):
         r = sssi_amd.Rtm2D(wl["V"], wl["VS"], wl["N"], wl["nB"], wl["d"], wl["d"], wl["dt"],
-                           wl["nt"], img_stride=5, ck_format=fmt)
-        for xb, xc, s in wl["shots"]:
-            r.add_shot(xb, xc, s, wl["wavelet"])
-        r.precompute_data(batched=True)
+                           wl["nt"], img_stride=wl["img_stride"], ck_format=fmt)
+        for xb, xc, sx in wl["shots"]:
+            r.add_shot(xb, xc, sx, wl["wavelet"])
+        t0 = time.perf_counter()
+        r.precompute_data(batched=batched)
+        t_true = time.perf_counter() - t0
         t0 = time.perf_counter()
         ms, launches = r.migrate(batched=batched)
         wall = time.perf_counter() - t0
-        res[name] = {"shots_per_hour": round(len(wl["shots"]) / wall * 3600, 1),
-                     "seconds": round(wall, 3), "event_ms": round(ms, 2), "launches": launches,
+        res[name] = {"shots_per_hour": round(nshot / wall * 3600, 1),
+                     "shots_per_hour_with_data_true": round(nshot / (wall + t_true) * 3600, 1),
+                     "seconds": round(wall, 3), "data_true_seconds": round(t_true, 3),
+                     "event_ms": round(ms, 2), "launches": launches,
                      "device_GB": round(r.device_bytes() / 1e9, 2)}
         r.close()
+    # served: each shot a job (extended window models), all submitted at once
+    nB = wl["nB"]
+    jobs = []
+    for xb, xc, sx in wl["shots"]:
+        jobs.append(dict(V=synth.extBoundary(wl["V"][:, xb:xb + xc], nB, 2),
+                         VS=synth.extBoundary(wl["VS"][:, xb:xb + xc], nB, 2), nt=wl["nt"],
+                         nDiffOrder=wl["N"], nBoundary=nB, dz=wl["d"], dx=wl["d"], dt=wl["dt"],
+                         src_ix=sx + nB, wavelet=wl["wavelet"], img_stride=wl["img_stride"],
+                         ck_format=args.ck))
+    with sssi_amd.Server(hbm_budget_bytes=args.budget_gb * 1e9, max_active=nshot) as s:
+        t0 = time.perf_counter()
+        ids = [s.submit_rtm2d(**kw) for kw in jobs]
+        s.drain()
+        wall = time.perf_counter() - t0
+        lat = np.array([s.info(i)["latency"] for i in ids])
+    res["served_" + args.ck] = {"shots_per_hour_with_data_true": round(nshot / wall * 3600, 1),
+                                "seconds": round(wall, 3),
+                                "p50_s": round(float(np.percentile(lat, 50)), 3),
+                                "p99_s": round(float(np.percentile(lat, 99)), 3)}
     b = res["batched_" + args.ck]["shots_per_hour"]
     p = res["plain_fp32"]["shots_per_hour"]
+    sv = res["served_" + args.ck]["shots_per_hour_with_data_true"]
+    pw = res["plain_fp32"]["shots_per_hour_with_data_true"]
     return {
         "metric": "RTM shots/hour (2D CPML, 64 shots, batched + 16-bit checkpoints)",
         "value": b, "unit": "shots/hour", "n_gpus": 1, "steps": 1, "warmup": 0,
@@ -185,7 +253,10 @@ def bench_rtm(args):
         "config": {"workload": "2D RTM, 64 shots, nz=600, nx~U[400,1200], nB=30, "
                                "nDiffOrder=4, nt=3000, imaging every 5 steps",
                    "parallelism": "1 GPU"},
-        "speedup_vs_plain_fp32": round(b / p, 3), "modes": res,
+        "speedup_vs_plain_fp32": round(b / p, 3),
+        "speedup_batching_alone": round(res["batched_fp32"]["shots_per_hour"] / p, 3),
+        "served_speedup_vs_plain_fp32_with_data_true": round(sv / pw, 3),
+        "modes": res,
     }
 
 
@@ -268,46 +339,24 @@ def bench_copy(args):
             "steps": args.steps}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="2d/3d: default kernel layout instead of the autotuned one")
-    ap.add_argument("--workload", default="2d",
-                    choices=["2d", "3d", "sponge", "rtm", "serve", "copy"],
-                    help="2d: the headline 2-D forward line (default); the others: other "
-                         "workloads")
-    ap.add_argument("--nt", type=int, default=None,
-                    help="3d / sponge: time steps (default 2000 / 4000)")
-    ap.add_argument("--shots", type=int, default=64, help="rtm: number of shots")
-    ap.add_argument("--ck", default="bf16", choices=["fp16", "bf16"], help="rtm: checkpoint type")
-    ap.add_argument("--jobs", type=int, default=60, help="serve: number of jobs")
-    ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate (1/s)")
-    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget (GB)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are "
-              "not part of this build.", file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "3d":
-        print(json.dumps(bench_3d(args)), flush=True)
-        return
-    if args.workload == "sponge":
-        print(json.dumps(bench_sponge(args)), flush=True)
-        return
-    if args.workload == "rtm":
-        print(json.dumps(bench_rtm(args)), flush=True)
-        return
-    if args.workload == "serve":
-        print(json.dumps(bench_serve(args)), flush=True)
-        return
-    if args.workload == "copy":
-        print(json.dumps(bench_copy(args)), flush=True)
-        return
+def _sub(fn, args, **over):
+    """Run one sub-benchmark with its own fixed step counts; a failure is
+    recorded instead of ending the run."""
+    a = argparse.Namespace(**vars(args))
+    for k, v in over.items():
+        setattr(a, k, v)
+    t0 = time.perf_counter()
+    try:
+        out = fn(a)
+    except Exception as e:  # noqa: BLE001 -- reported in the JSON line
+        out = {"error": f"{type(e).__name__}: {e}"}
+    out["bench_wall_s"] = round(time.perf_counter() - t0, 1)
+    print(f"[bench] {fn.__name__}: {out.get('value')} {out.get('unit', '')} "
+          f"({out['bench_wall_s']} s)", file=sys.stderr, flush=True)
+    return out
 
+
+def bench_2d(args):
     import sssi_amd
     wl = workload()
     V = wl["V"]
@@ -334,9 +383,9 @@ def main():
     P.sync()
     wall = time.perf_counter() - t0
     data = P.data(nt)
+    P.close()
     if not np.all(np.isfinite(data)):
-        print("non-finite receiver data", file=sys.stderr)
-        sys.exit(3)
+        raise RuntimeError("non-finite receiver data")
 
     updates = cells * nt * args.steps
     value = updates / wall / 1e9
@@ -369,6 +418,48 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="default workload: the headline 2-D line only, without the config "
+                         "3 / 4 / 5 sub-benchmarks")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="2d/3d: default kernel layout instead of the autotuned one")
+    ap.add_argument("--workload", default="2d",
+                    choices=["2d", "3d", "sponge", "rtm", "serve", "copy"],
+                    help="2d: the headline 2-D forward line with the sub-benchmarks (default); "
+                         "the others: one workload alone")
+    ap.add_argument("--nt", type=int, default=None,
+                    help="3d / sponge: time steps (default 2000 / 4000)")
+    ap.add_argument("--shots", type=int, default=64, help="rtm: number of shots")
+    ap.add_argument("--ck", default="bf16", choices=["fp16", "bf16"], help="rtm: checkpoint type")
+    ap.add_argument("--jobs", type=int, default=60, help="serve: number of jobs")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate (1/s)")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget (GB)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are "
+              "not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    one = {"3d": bench_3d, "sponge": bench_sponge, "rtm": bench_rtm, "serve": bench_serve,
+           "copy": bench_copy}
+    if args.workload in one:
+        print(json.dumps(one[args.workload](args)), flush=True)
+        return
+    out = bench_2d(args)
+    print(f"[bench] headline: {out['value']} Gcell-updates/s", file=sys.stderr, flush=True)
+    if not args.no_sub:
+        # configs 4, 3 and 5, each with its own fixed step counts
+        out["cfg4_3d"] = _sub(bench_3d, args, steps=2, warmup=1, nt=None)
+        out["cfg3_rtm"] = _sub(bench_rtm, args, shots=64)
+        out["cfg5_serve"] = _sub(bench_serve, args, jobs=60, rate=20.0)
     print(json.dumps(out), flush=True)
 
 
