@@ -273,6 +273,16 @@ int sssi_server_submit_rtm2d(sssi_server *s, const double *V, const double *VS, 
                              int64_t nx, int64_t nt, int nDiffOrder, int nBoundary, double dz,
                              double dx, double dt, int src_ix, const double *wavelet,
                              int img_stride, int ck_format, double eps, int64_t *job_id);
+/* An RTM shot whose observed data dataTrue (nx, nt) column-major is already
+ * known -- the reference's apps cache it per shot and load it instead of
+ * recomputing (applications/rtm_2d_acoustic/mainRtmTimeCpmlFor2dAw.m:170-184):
+ * the job skips the fwd(V) propagation.  data_true = NULL computes it, as
+ * sssi_server_submit_rtm2d does. */
+int sssi_server_submit_rtm2d_data(sssi_server *s, const double *V, const double *VS, int64_t nz,
+                                  int64_t nx, int64_t nt, int nDiffOrder, int nBoundary,
+                                  double dz, double dx, double dt, int src_ix,
+                                  const double *wavelet, const double *data_true, int img_stride,
+                                  int ck_format, double eps, int64_t *job_id);
 int sssi_server_submit_fwd3d(sssi_server *s, const double *v, int64_t nz, int64_t nx, int64_t ny,
                              int64_t nt, int nDiffOrder, int nBoundary, double dz, double dx,
                              double dy, double dt, int src_iz, int src_ix, int src_iy,

@@ -7,6 +7,9 @@
 namespace sssi {
 
 static thread_local std::string g_last_error;
+static thread_local hipStream_t g_alloc_stream = nullptr;
+
+hipStream_t &alloc_stream() { return g_alloc_stream; }
 
 void set_last_error(const char *msg) { g_last_error = msg ? msg : ""; }
 

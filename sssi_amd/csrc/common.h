@@ -46,33 +46,62 @@ const char *last_error_text();
                          __FILE__, __LINE__);                                           \
     } while (0)
 
+// Stream of the stream-ordered allocator for the calling thread's device
+// allocations (hipMallocAsync / hipFreeAsync from the device's memory pool),
+// or null for hipMalloc / hipFree.  The serving loop's helper threads set it:
+// hipFree waits for the whole device, which would stall every resident job
+// each time one finishes.
+hipStream_t &alloc_stream();
+
 // Owning device buffer.
 template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    hipStream_t as = nullptr;  // allocated on this stream's pool (null: hipMalloc)
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n), as(o.as) { o.p = nullptr; o.n = 0; }
     DevBuf &operator=(DevBuf &&o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; as = o.as;
+            o.p = nullptr; o.n = 0;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) SSSI_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+        as = alloc_stream();
+        if (!count) return;
+        if (as) SSSI_HIP(hipMallocAsync((void **)&p, count * sizeof(T), as));
+        else SSSI_HIP(hipMalloc((void **)&p, count * sizeof(T)));
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            // freed on the releasing thread's allocation stream if it has one
+            // (the memory's users have finished), else on the allocating one
+            if (as) (void)hipFreeAsync(p, alloc_stream() ? alloc_stream() : as);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
     size_t bytes() const { return n * sizeof(T); }
 };
+
+// v (column-major FP64, z fastest: v[(k*nx + j)*nz + i]) -> a padded FP32
+// device field: dst[(i + hz)*plane + (k + hy)*pitch + j + hx] = (v dt)^2 when
+// square, else v dt, each rounded once from FP64 (the arithmetic of a host
+// conversion).  The FP64 model goes up in chunks through a device staging
+// buffer and is converted there; the rest of dst is zero-filled first.
+void upload_model(const double *v, int64_t nz, int64_t nx, int64_t ny, double dt, bool square,
+                  float *dst, size_t dst_n, int64_t plane, int64_t pitch, int hz, int hy, int hx,
+                  hipStream_t s);
 
 // Owning pinned host buffer (hipHostMalloc): copies between it and the device
 // run at link speed rather than through the runtime's pageable staging.

@@ -48,7 +48,7 @@ struct Job2D {
     // checkpoints of the forward field at t % ck_stride == 0 (slot c = t/ck_stride)
     int ck_stride, ck_fmt;
     float *ck32;                  // kCkF32: [c][nz*nx]
-    float *ck_stage;              // kCkF16/kCkBF16: FP32 staging [c&1][nz*nx]
+    float *ck_stage;              // kCkF16: FP32 staging [c&1][nz*nx]
     uint16_t *ck16;               // [c][nz*nx], value = half(p / scale[c])
     unsigned *ck_max;             // [c][kCkSlots] bits of max |p| (per wave, atomicMax)
     float *ck_scale;              // [c]
@@ -99,8 +99,8 @@ class Cpml2dGrid {
     // tile block covered by the interior march (not launched as a tile block)
     bool block_in_march(int tile) const;
     int n_march_blocks() const { return mntx > 0 ? (mz_rows / (4 * ng * rz)) * mntx : 0; }
-    // zero the wavefield and memory variables and re-arm the watchdog
-    void zero_state(hipStream_t s);
+    // zero the wavefield and memory variables and (arm) re-arm the watchdog
+    void zero_state(hipStream_t s, bool arm = true);
     // scan level `which` for NaN/Inf (reported into health as `tag`)
     void scan_nonfinite(int which, int tag, hipStream_t s) const;
     // descriptor with geometry, coefficients and state pointers filled in
@@ -173,7 +173,8 @@ class StepGraph {
     DevBuf<int> base_;
 };
 
-// Pack the staged FP32 checkpoint `c` of a job into 16-bit (after its last step).
+// Pack the staged FP32 checkpoint `c` of a job into FP16 (after its last step;
+// nothing to do for FP32 and BF16 checkpoints, which are written directly).
 void launch_ck_pack(const Job2D &job, int c, hipStream_t s);
 
 }  // namespace sssi

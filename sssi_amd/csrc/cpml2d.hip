@@ -53,13 +53,29 @@ __device__ __forceinline__ float f16_to_32(uint16_t h, int fmt) {
 }
 
 // Wave-uniform quantities of one time step of one job.
+//
+// 16-bit checkpoints carry one FP32 scale each.  BF16 has FP32's exponent
+// range, so its checkpoint c is written directly in its own step with the scale
+// taken from checkpoint c-1's |p| maximum (complete: written by an earlier
+// launch; 1 for c = 0) -- one 2-byte store per cell.  FP16 would overflow a
+// field that grows past twice the previous maximum, so its checkpoint c is
+// staged in FP32 and packed during the next step (or after the last one) with
+// its own maximum.
 struct StepCtx {
     int t, par;
     uint32_t rowb, nbytes;
-    bool snap_now, ck_now, stage_now, pack_now, im_now;
+    bool snap_now, ck_now, direct_now, stage_now, pack_now, im_now;
     int c, ci_im;
     float scale, inv_scale, im_scale;
 };
+
+__device__ __forceinline__ float ck_scale_of(const Job2D &J, int c) {
+    const unsigned *mx = J.ck_max + (size_t)c * kCkSlots;
+    unsigned m = 0;
+    for (int q = 0; q < kCkSlots; ++q) m = max(m, ld_uniform(mx + q));
+    const float amax = __uint_as_float(m);
+    return amax > 0.f ? amax * (1.0f / 32768.0f) : 1.0f;
+}
 
 __device__ __forceinline__ StepCtx step_ctx(const Job2D &J, int step) {
     StepCtx S;
@@ -71,19 +87,15 @@ __device__ __forceinline__ StepCtx step_ctx(const Job2D &J, int step) {
     S.snap_now = J.snap && (S.t % J.snap_stride == 0);
     S.ck_now = J.ck_fmt != kCkNone && (S.t % J.ck_stride == 0);
     S.c = S.ck_now ? S.t / J.ck_stride : 0;
-    S.stage_now = S.ck_now && J.ck_fmt != kCkF32;
+    S.direct_now = S.ck_now && J.ck_fmt == kCkBF16;
+    S.stage_now = S.ck_now && J.ck_fmt == kCkF16;
     S.pack_now = S.stage_now && S.c > 0;
     S.im_now = J.im_num && (S.t % J.im_stride == 0);
     S.ci_im = S.im_now ? S.t / J.im_stride : 0;
     S.scale = S.inv_scale = S.im_scale = 1.f;
-    if (S.pack_now) {
-        // scale of checkpoint c-1 from its per-wave maxima (complete: written
-        // by the previous launch)
-        const unsigned *mx = J.ck_max + (size_t)(S.c - 1) * kCkSlots;
-        unsigned m = 0;
-        for (int q = 0; q < kCkSlots; ++q) m = max(m, ld_uniform(mx + q));
-        const float amax = __uint_as_float(m);
-        S.scale = amax > 0.f ? amax * (1.0f / 32768.0f) : 1.0f;
+    if (S.pack_now || (S.direct_now && S.c > 0)) {
+        // pack: checkpoint c-1's own scale; direct: checkpoint c scaled by c-1's
+        S.scale = ck_scale_of(J, S.c - 1);
         S.inv_scale = 1.0f / S.scale;
     }
     if (S.im_now && J.im_fmt != kCkF32) S.im_scale = ld_uniform(J.im_scale + S.ci_im);
@@ -145,6 +157,9 @@ __device__ __forceinline__ void tile_update(const Job2D &J, const StepCtx &S,
             if (S.ck_now) {
                 if (J.ck_fmt == kCkF32) {
                     J.ck32[(size_t)S.c * ncell + cell] = pn;
+                } else if (S.direct_now) {
+                    J.ck16[(size_t)S.c * ncell + cell] = f32_to_16(pn * S.inv_scale, J.ck_fmt);
+                    wmax = fmaxf(wmax, fabsf(pn));
                 } else {
                     J.ck_stage[(size_t)(S.c & 1) * ncell + cell] = pn;
                     wmax = fmaxf(wmax, fabsf(pn));
@@ -175,10 +190,10 @@ __device__ __forceinline__ void pin(float (&v)[R]) {
     for (int r = 0; r < R; ++r) asm volatile("" : "+v"(v[r]));
 }
 
-// Per-wave |p| maximum of a staged checkpoint into one of the kCkSlots slots.
+// Per-wave |p| maximum of a 16-bit checkpoint into one of the kCkSlots slots.
 __device__ __forceinline__ void ck_max_commit(const Job2D &J, const StepCtx &S, float wmax,
                                               int slot, int lane) {
-    if (!S.stage_now) return;
+    if (!S.stage_now && !S.direct_now) return;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, off, 64));
     if (lane == 0)
@@ -468,7 +483,10 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
     tile_sources<RZ>(J, t, zs, txi, j, jin, s);
 #pragma unroll
     for (int r = 0; r < RZ; ++r) pc[r] = col[L + r];
-    if (S.pack_now && tile == 0 && w == 0 && lane == 0) J.ck_scale[S.c - 1] = S.scale;
+    if (tile == 0 && w == 0 && lane == 0) {
+        if (S.pack_now) J.ck_scale[S.c - 1] = S.scale;
+        if (S.direct_now) J.ck_scale[S.c] = S.scale;
+    }
     float wmax = 0.f, pn0 = 0.f;
     tile_update<RZ>(J, S, rO, vcol, srow, zs, j, jin, pc, po, v2, lap, s, wmax, pn0);
     ck_max_commit(J, S, wmax, tile * 4 + w, lane);
@@ -569,7 +587,10 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
                         wmax, pn0);
         if (J.health && zs == 0) report_nonfinite(J.health, nonfinite(pn0), step);
     }
-    if (S.pack_now && mb == 0 && w == 0 && lane == 0) J.ck_scale[S.c - 1] = S.scale;
+    if (mb == 0 && w == 0 && lane == 0) {
+        if (S.pack_now) J.ck_scale[S.c - 1] = S.scale;
+        if (S.direct_now) J.ck_scale[S.c] = S.scale;
+    }
     ck_max_commit(J, S, wmax, mb * 4 + w, lane);
 }
 
@@ -708,24 +729,8 @@ Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB
     vdt2.alloc(np);
     health.alloc(1);
     SSSI_HIP(hipMemset(health.p, 0x7f, sizeof(int)));
-    {
-        // v (column-major, z fastest) -> padded x-fastest rows, zero halo.  The
-        // zero fill and the transpose are split over host threads; the
-        // transpose reads each column contiguously and writes one word into
-        // each of nz rows, which stay in cache from one column to the next.
-        HostBuf<float> hv(np);
-        parallel_ranges((int64_t)np, [&](int64_t a, int64_t b) {
-            std::fill(hv.p + a, hv.p + b, 0.f);
-        }, 1 << 18);
-        parallel_ranges((int64_t)nx, [&](int64_t j0, int64_t j1) {
-            for (int64_t j = j0; j < j1; ++j)
-                for (int64_t i = 0; i < nz; ++i) {
-                    const double vv = v[j * nz + i];
-                    hv.p[(size_t)(i + kHZ) * pitch + j + kHX] = (float)((vv * dt) * (vv * dt));
-                }
-        }, 16);
-        SSSI_HIP(hipMemcpy(vdt2.p, hv.p, np * sizeof(float), hipMemcpyHostToDevice));
-    }
+    // v^2 dt^2 in the padded layout, converted on the device from the FP64 model
+    upload_model(v, nz, nx, 1, dt, true, vdt2.p, vdt2.n, pitch, 0, kHZ, 0, kHX, alloc_stream());
     stencil_2d(N, dz, dx, coef);
     if (nB > 0) {
         std::vector<float> zb, xb;
@@ -801,10 +806,10 @@ int march_groups(int rz) {
     return (ng >= 2 && ng <= 4) || ng == 8 ? ng : 0;
 }
 
-void Cpml2dGrid::zero_state(hipStream_t s) {
+void Cpml2dGrid::zero_state(hipStream_t s, bool arm) {
     for (auto *b : {&p[0], &p[1], &zphi[0], &zphi[1], &zpsi, &xphi[0], &xphi[1], &xpsi})
         if (b->p) SSSI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), s));
-    arm_health(health.p, s);
+    if (arm) arm_health(health.p, s);
 }
 
 void Cpml2dGrid::scan_nonfinite(int which, int tag, hipStream_t s) const {
@@ -998,6 +1003,7 @@ void launch_cpml2d_batch(int N, int rz, int ng, const Job2D *jobs, const int2 *o
 }
 
 void launch_ck_pack(const Job2D &job, int c, hipStream_t s) {
+    if (job.ck_fmt != kCkF16) return;  // only FP16 checkpoints are staged
     const size_t n = (size_t)job.nz * job.nx;
     hipLaunchKernelGGL(ck_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        job.ck_stage + (size_t)(c & 1) * n, job.ck16 + (size_t)c * n,
@@ -1006,6 +1012,54 @@ void launch_ck_pack(const Job2D &job, int c, hipStream_t s) {
 }
 
 bool graphs_enabled() { return env_int("SSSI_GRAPH", 0) != 0; }
+
+// Columns c0 .. c0+ncol-1 (column c = k*nx + j) of the FP64 model in the
+// staging buffer -> the padded FP32 field.
+__global__ void model_to_padded(const double *__restrict__ vc, int64_t nz, int64_t nx,
+                                int64_t c0, int64_t ncol, double dt, int square,
+                                float *__restrict__ dst, int64_t plane, int64_t pitch, int hz,
+                                int hy, int hx) {
+    const size_t n = (size_t)nz * ncol;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int64_t c = c0 + (int64_t)(q / nz), i = (int64_t)(q % nz);
+        const int64_t k = c / nx, j = c - k * nx;
+        const double t = vc[q] * dt;
+        dst[(size_t)(i + hz) * plane + (size_t)(k + hy) * pitch + (j + hx)] =
+            (float)(square ? t * t : t);
+    }
+}
+
+__global__ void fill_zero_f32(float *p, size_t n) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x)
+        p[q] = 0.f;
+}
+
+void upload_model(const double *v, int64_t nz, int64_t nx, int64_t ny, double dt, bool square,
+                  float *dst, size_t dst_n, int64_t plane, int64_t pitch, int hz, int hy, int hx,
+                  hipStream_t s) {
+    auto blocks = [](size_t n) {
+        return (unsigned)std::max<size_t>(1, std::min<size_t>(4096, (n + 255) / 256));
+    };
+    hipLaunchKernelGGL(fill_zero_f32, dim3(blocks(dst_n)), dim3(256), 0, s, dst, dst_n);
+    SSSI_HIP(hipGetLastError());
+    const int64_t ncols = nx * ny;
+    const int64_t per = std::max<int64_t>(1, ((int64_t)4 << 20) / nz);  // <= 32 MB chunks
+    const int64_t chunk = std::min(per, ncols);
+    DevBuf<double> stage((size_t)chunk * nz);
+    for (int64_t c0 = 0; c0 < ncols; c0 += chunk) {
+        const int64_t nc = std::min(chunk, ncols - c0);
+        // pageable source: the copy has left the host buffer when the call returns
+        SSSI_HIP(hipMemcpyAsync(stage.p, v + c0 * nz, (size_t)nc * nz * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(model_to_padded, dim3(blocks((size_t)nc * nz)), dim3(256), 0, s,
+                           stage.p, nz, nx, c0, nc, dt, square ? 1 : 0, dst, plane, pitch, hz, hy,
+                           hx);
+        SSSI_HIP(hipGetLastError());
+    }
+    SSSI_HIP(hipStreamSynchronize(s));  // the staging buffer is released on return
+}
 
 StepGraph::~StepGraph() { clear(); }
 

@@ -169,19 +169,7 @@ class Prop2D final : public Prop {
             sp[0].alloc(np);
             sp[1].alloc(np);
             vdt.alloc(np);
-            {
-                // as the CPML grid's upload: threaded zero fill and transpose
-                HostBuf<float> hv(np);
-                parallel_ranges((int64_t)np, [&](int64_t a, int64_t b) {
-                    std::fill(hv.p + a, hv.p + b, 0.f);
-                }, 1 << 18);
-                parallel_ranges(nx, [&](int64_t j0, int64_t j1) {
-                    for (int64_t j = j0; j < j1; ++j)
-                        for (int64_t i = 0; i < nz; ++i)
-                            hv.p[(size_t)(i + kHZ) * pitch + j + kHX] = (float)(v[j * nz + i] * dt);
-                }, 16);
-                SSSI_HIP(hipMemcpy(vdt.p, hv.p, np * sizeof(float), hipMemcpyHostToDevice));
-            }
+            upload_model(v, nz, nx, 1, dt, false, vdt.p, vdt.n, pitch, 0, kHZ, 0, kHX, stream);
             std::vector<float> gg(std::max(nB, 1), 1.f);
             const double alpha = 0.015;
             for (int m = 0; m < nB; ++m) {

@@ -703,26 +703,9 @@ class Prop3D final : public Prop {
         p[0].alloc(np);
         p[1].alloc(np);
         vfield.alloc(np);
-        {
-            // v (z fastest) -> padded [z][y][x], zero halo.  The zero fill and
-            // the transpose are split over host threads; the transpose reads
-            // each (j, k) column contiguously and writes one word into each of
-            // nz plane rows, which stay in cache from one column to the next.
-            HostBuf<float> hv(np);
-            parallel_ranges((int64_t)np, [&](int64_t a, int64_t b) {
-                std::fill(hv.p + a, hv.p + b, 0.f);
-            }, 1 << 18);
-            parallel_ranges(ny, [&](int64_t k0, int64_t k1) {
-                for (int64_t k = k0; k < k1; ++k)
-                    for (int64_t j = 0; j < nx; ++j)
-                        for (int64_t i = 0; i < nz; ++i) {
-                            const double vv = v[(k * nx + j) * nz + i];
-                            hv.p[((size_t)(i + kHZ) * npy + k + kHY) * pitch + j + kHX] =
-                                (float)((vv * dt) * (vv * dt));
-                        }
-            }, 1);
-            SSSI_HIP(hipMemcpy(vfield.p, hv.p, np * sizeof(float), hipMemcpyHostToDevice));
-        }
+        // v^2 dt^2 in the padded [z][y][x] layout, converted on the device
+        upload_model(v, nz, nx, ny, dt, true, vfield.p, vfield.n, pitch * npy, pitch, kHZ, kHY,
+                     kHX, stream);
         double c[8];
         staggered_coefficients(N, c);
         for (int q = 0; q < N; ++q) {

@@ -122,7 +122,7 @@ class Rtm2D {
         if (fmt == kCkF32) {
             s->ck32.alloc((size_t)nck() * ncell);
         } else {
-            s->stage.alloc(2 * ncell);
+            if (fmt == kCkF16) s->stage.alloc(2 * ncell);
             s->ck16.alloc((size_t)nck() * ncell);
         }
         s->ckmax.alloc((size_t)nck() * kCkSlots);
@@ -175,7 +175,7 @@ class Rtm2D {
         SSSI_HIP(hipMemsetAsync(s.den.p, 0, s.den.bytes(), stream));
     }
     void finish_fwd(RtmShot &s, const Job2D &J) {
-        if (fmt != kCkF32) {
+        if (fmt == kCkF16) {
             launch_ck_pack(J, (int)((nt - 1) / stride), stream);
             launches++;
         }

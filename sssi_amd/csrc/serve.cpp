@@ -2,14 +2,20 @@
 // continuous batching on one GPU.
 //
 // Jobs (2-D CPML forward, 2-D CPML RTM shot, 3-D CPML forward) arrive on a
-// queue at any time.  A worker thread advances all resident jobs one time
-// step per tick: the 2-D jobs of each nDiffOrder share one launch (the
-// batched step kernel, every job at its own time index), 3-D jobs get one
-// launch each.  A job is admitted when a slot is free and its device
-// footprint fits the HBM budget; it joins the next tick, its wavefields,
-// memory variables and checkpoints stay resident until it finishes, and its
-// memory is released as soon as its results are copied out.  A job's
-// arithmetic does not depend on its co-residents (per-job state, no
+// queue at any time.  Helper threads build an admitted job's device state
+// (uploads and conversions on their own streams, stream-ordered allocation, so
+// no call waits for the whole device) while a worker thread keeps the resident
+// jobs moving in ticks.  A tick issues, on each of four equal-priority streams,
+// a chunk of time steps: the 2-D jobs of one layout (nDiffOrder, tile height,
+// march groups) share one batched launch per step, every job at its own time
+// index, on that layout's stream; each 3-D job steps on one of two 3-D
+// streams.  The streams run concurrently on the device, so a large 3-D job
+// does not set the step rate of the 2-D jobs, and the worker keeps at most two
+// ticks in flight, so a job that becomes ready joins within two ticks.  A job
+// is admitted when a slot is free and its device footprint fits the HBM
+// budget; its wavefields, memory variables and checkpoints stay resident until
+// it finishes, and a helper copies its result out and releases its memory.  A
+// job's arithmetic does not depend on its co-residents (per-job state, no
 // cross-job reductions), so a result is identical alone and in the mix.
 #include <hip/hip_runtime.h>
 
@@ -50,6 +56,7 @@ struct ServeJob {
     double t_arrive = 0, t_admit = 0, t_done = 0;
     // inputs
     std::vector<double> v, vs, wavelet;
+    std::vector<float> dtrue;  // RTM: cached dataTrue [t][j], or empty (computed)
     int64_t nz = 0, nx = 0, ny = 1, nt = 0;
     int N = 1, nB = 0, src_iz = 0, src_ix = 0, src_iy = 0, img_stride = 1, ck_fmt = kCkF32;
     double dz = 0, dx = 0, dy = 0, dt = 0, eps = 1e-3;
@@ -63,11 +70,16 @@ struct ServeJob {
     DevBuf<float> ckscale;
     // device state (3-D)
     std::unique_ptr<Prop> p3;
-    // schedule
+    // schedule (worker)
+    bool have_true = false;  // RTM: dataTrue given at submission
+    int lane = 0;           // work stream
     int phase = 0;          // RTM: 0 dataTrue, 1 fwd(VS)+checkpoints, 2 rvs(V)+imaging
-    int64_t phase_start = 0;
+    int64_t steps = 0;      // steps issued in the current phase
+    bool started = false;
     Job2D job{};
     // results
+    const float *res_dev = nullptr;  // device result the helper copies out
+    size_t res_n = 0;
     std::vector<float> out;
     int health_h[2] = {kHealthy, kHealthy};  // watchdog words, copied with the result
     hipEvent_t done_ev = nullptr;
@@ -77,44 +89,62 @@ struct ServeJob {
 
 class Server {
    public:
+    static constexpr int kLanes = 4;    // work streams: 0-1 2-D layouts, 2-3 3-D jobs
+    static constexpr int kHelpers = 4;  // threads building and releasing jobs
+    static constexpr int kChunk2 = 16;  // time steps per tick of a 2-D layout
+    static constexpr int kChunk3 = 4;   // time steps per tick of a 3-D job
+    static constexpr int kAhead = 2;    // ticks in flight
+
     double budget = 0;
     int max_active = 1;
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t lanes[kLanes] = {};
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_helper;
     std::deque<ServeJob *> queue;       // submitted, waiting for a slot and memory
-    std::deque<ServeJob *> ready;       // device state built, waiting for the next tick
-    std::deque<ServeJob *> finished;    // results copied out, device state to release
+    std::deque<ServeJob *> ready;       // device state built, waiting for the worker
+    std::deque<ServeJob *> finished;    // GPU work complete: copy the result, release
     std::map<int64_t, std::unique_ptr<ServeJob>> jobs;
-    std::vector<ServeJob *> active;
+    std::vector<ServeJob *> active;     // owned by the worker
     int reserved_slots = 0;             // jobs admitted (being built, ready or active)
-    std::condition_variable cv_helper;
-    std::thread worker, helper;
+    std::vector<std::thread> helpers;
+    std::thread worker;
     std::atomic<bool> stop{false};
     int64_t next_id = 1;
     int64_t tick = 0;
     size_t used = 0;
     double cell_updates = 0, busy_s = 0;
     int64_t launches = 0;
-    bool groups_dirty = true;
     std::string fatal;
 
+    // 2-D jobs of one layout share a batched launch per step on the layout's
+    // lane; `step` counts the layout's launches (a member's time index is
+    // step - its phase's start step)
     struct Group {
-        int N, rz, ng;
+        int N, rz, ng, lane;
+        int step = 0;
+        bool dirty = true;
         std::vector<ServeJob *> members;
         DevBuf<Job2D> desc;
         DevBuf<int2> order;
         int nblk = 0;
     };
-    std::vector<std::unique_ptr<Group>> groups;
-    std::vector<std::unique_ptr<Group>> retired;  // freed after the next sync
+    std::map<std::tuple<int, int, int>, std::unique_ptr<Group>> groups;
+    std::deque<std::vector<hipEvent_t>> inflight;  // per tick: one event per used lane
+    std::vector<hipEvent_t> spare_events;
 
     Server(double budget_bytes, int max_active_) : budget(budget_bytes), max_active(max_active_) {
         SSSI_HIP(hipGetDevice(&device));
-        SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto &l : lanes) SSSI_HIP(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
+        // keep freed job memory in the device pool for the next jobs instead of
+        // returning it at every synchronisation
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
         worker = std::thread([this] { loop(); });
-        helper = std::thread([this] { helper_loop(); });
+        for (int h = 0; h < kHelpers; ++h) helpers.emplace_back([this] { helper_loop(); });
     }
 
     ~Server() {
@@ -125,14 +155,23 @@ class Server {
         cv_work.notify_all();
         cv_helper.notify_all();
         if (worker.joinable()) worker.join();
-        if (helper.joinable()) helper.join();
-        for (auto &kv : jobs)
-            if (kv.second->done_ev) (void)hipEventDestroy(kv.second->done_ev);
-        active.clear();
+        for (auto &h : helpers)
+            if (h.joinable()) h.join();
+        for (auto &l : lanes)
+            if (l) (void)hipStreamSynchronize(l);
+        // the helpers' streams are gone: free in order on a lane instead
+        alloc_stream() = lanes[0];
+        for (auto &kv : jobs) release(*kv.second);
         groups.clear();
-        retired.clear();
+        active.clear();
         jobs.clear();
-        if (stream) (void)hipStreamDestroy(stream);
+        alloc_stream() = nullptr;
+        for (auto &t : inflight)
+            for (hipEvent_t e : t) (void)hipEventDestroy(e);
+        for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
+        (void)hipDeviceSynchronize();  // stream-ordered frees above
+        for (auto &l : lanes)
+            if (l) (void)hipStreamDestroy(l);
     }
 
     int64_t submit(std::unique_ptr<ServeJob> j) {
@@ -165,7 +204,7 @@ class Server {
             const size_t cells = (size_t)j.nz * j.nx;
             const size_t nck = (j.nt + j.img_stride - 1) / j.img_stride;
             const size_t ck = nck * cells * (j.ck_fmt == kCkF32 ? 4 : 2) +
-                              (j.ck_fmt == kCkF32 ? 0 : 2 * cells * 4);
+                              (j.ck_fmt == kCkF16 ? 2 * cells * 4 : 0);
             return 2 * grid2(j.nz, j.nx, j.nB) + 2 * (size_t)j.nt * j.nx * 4 + ck +
                    3 * cells * 4 + nck * (kCkSlots + 1) * 4;
         }
@@ -177,12 +216,15 @@ class Server {
         return 3 * np * 4 + 3 * strips * 4 + (size_t)j.nt * j.nx * j.ny * 4;
     }
 
-    // ---- helper thread: builds device state of admitted jobs (host-side
-    // preparation and uploads) and releases the state of finished ones, so the
-    // worker's launch stream never waits on them.
+    // ---- helpers: build the device state of admitted jobs; copy the results
+    // of completed ones out and release their memory.  Allocations and frees
+    // go through the stream-ordered pool on the helper's own stream.
     void helper_loop() {
+        hipStream_t hs = nullptr;
         try {
             SSSI_HIP(hipSetDevice(device));
+            SSSI_HIP(hipStreamCreateWithFlags(&hs, hipStreamNonBlocking));
+            alloc_stream() = hs;
             while (true) {
                 ServeJob *build = nullptr, *drop = nullptr;
                 {
@@ -204,8 +246,11 @@ class Server {
                     }
                 }
                 if (drop) {
+                    deliver(*drop, hs);
                     release(*drop);
+                    SSSI_HIP(hipStreamSynchronize(hs));
                     std::lock_guard<std::mutex> lk(mu);
+                    drop->t_done = now_s();  // the result is on the host
                     used -= drop->bytes;
                     reserved_slots--;
                     drop->state = drop->error.empty() ? kDone : kFailed;
@@ -214,7 +259,8 @@ class Server {
                     continue;
                 }
                 try {
-                    setup(*build);
+                    setup(*build, hs);
+                    SSSI_HIP(hipStreamSynchronize(hs));
                     std::lock_guard<std::mutex> lk(mu);
                     build->t_admit = now_s();
                     ready.push_back(build);
@@ -227,12 +273,44 @@ class Server {
                     used -= build->bytes;
                     reserved_slots--;
                     cv_done.notify_all();
+                    cv_helper.notify_all();
                 }
             }
         } catch (const std::exception &e) {
             std::lock_guard<std::mutex> lk(mu);
             fatal = e.what();
             cv_done.notify_all();
+        }
+        alloc_stream() = nullptr;
+        if (hs) {
+            (void)hipStreamSynchronize(hs);
+            (void)hipStreamDestroy(hs);
+        }
+    }
+
+    // result and watchdog words to the host (the job's GPU work is complete)
+    void deliver(ServeJob &j, hipStream_t hs) {
+        if (j.res_dev) {
+            j.out.resize(j.res_n);
+            SSSI_HIP(hipMemcpyAsync(j.out.data(), j.res_dev, j.res_n * sizeof(float),
+                                    hipMemcpyDeviceToHost, hs));
+        }
+        const int *hw[2] = {j.kind == kFwd3d ? (j.p3 ? j.p3->health_word() : nullptr)
+                                             : (j.g ? j.g->health.p : nullptr),
+                            j.gs ? j.gs->health.p : nullptr};
+        for (int q = 0; q < 2; ++q)
+            if (hw[q])
+                SSSI_HIP(hipMemcpyAsync(&j.health_h[q], hw[q], sizeof(int), hipMemcpyDeviceToHost,
+                                        hs));
+        SSSI_HIP(hipStreamSynchronize(hs));
+        if (j.health_h[0] != kHealthy || j.health_h[1] != kHealthy) {
+            char msg[256];
+            snprintf(msg, sizeof(msg),
+                     "non-finite wavefield (NaN/Inf) at step %d: the time step is likely beyond "
+                     "the scheme's stability limit",
+                     std::min(j.health_h[0], j.health_h[1]) + 1);
+            j.error = msg;
+            j.code = SSSI_EUNSTABLE;
         }
     }
 
@@ -246,57 +324,20 @@ class Server {
         j.ckmax.release();
         j.src = DevSources();
         j.src_s = DevSources();
+        j.res_dev = nullptr;
         if (j.done_ev) {
             (void)hipEventDestroy(j.done_ev);
             j.done_ev = nullptr;
         }
     }
 
-    // ---- worker: one tick = one time step of every resident job
-    void loop() {
-        try {
-            SSSI_HIP(hipSetDevice(device));
-            while (true) {
-                {
-                    std::unique_lock<std::mutex> lk(mu);
-                    cv_work.wait(lk, [&] { return stop || !ready.empty() || !active.empty(); });
-                    if (stop && active.empty()) break;
-                    while (!ready.empty()) {
-                        ServeJob *j = ready.front();
-                        ready.pop_front();
-                        j->state = kActive;
-                        active.push_back(j);
-                        groups_dirty = true;
-                    }
-                }
-                for (ServeJob *j : active)
-                    if (j->state == kActive && j->phase_start < 0) start_phase(*j);
-                if (active.empty()) continue;
-                const double t0 = now_s();
-                step_all();
-                finish_phases();
-                busy_s += now_s() - t0;
-            }
-        } catch (const std::exception &e) {
-            std::lock_guard<std::mutex> lk(mu);
-            fatal = e.what();
-            for (auto &kv : jobs)
-                if (kv.second->state != kDone) {
-                    kv.second->state = kFailed;
-                    kv.second->error = fatal;
-                }
-            cv_done.notify_all();
-        }
-    }
-
-    void setup(ServeJob &j) {
+    void setup(ServeJob &j, hipStream_t hs) {
         if (j.kind == kFwd3d) {
             j.p3.reset(make_prop3d(j.v.data(), j.nz, j.nx, j.ny, j.N, j.nB, j.dz, j.dx, j.dy, j.dt));
-            j.p3->use_stream(stream);
             j.p3->set_points(1, &j.src_iz, &j.src_ix, &j.src_iy, j.wavelet.data(), j.nt);
             j.p3->ensure_rec(j.nt, j.nx * j.ny);
             j.p3->sync();
-            j.phase_start = -1;
+            j.lane = 2 + (int)(j.id & 1);
             return;
         }
         j.g.reset(new Cpml2dGrid(j.v.data(), j.nz, j.nx, j.N, j.nB, j.dz, j.dx, j.dt));
@@ -311,7 +352,7 @@ class Server {
             if (j.ck_fmt == kCkF32) {
                 j.ck32.alloc(nck * cells);
             } else {
-                j.stage.alloc(2 * cells);
+                if (j.ck_fmt == kCkF16) j.stage.alloc(2 * cells);
                 j.ck16.alloc(nck * cells);
             }
             j.ckmax.alloc(nck * kCkSlots);
@@ -319,25 +360,87 @@ class Server {
             j.num.alloc(cells);
             j.den.alloc(cells);
             j.img.alloc(cells);
+            if (!j.dtrue.empty()) {
+                SSSI_HIP(hipMemcpyAsync(j.rec.p, j.dtrue.data(), j.rec.bytes(),
+                                        hipMemcpyHostToDevice, hs));
+                SSSI_HIP(hipStreamSynchronize(hs));
+                std::vector<float>().swap(j.dtrue);
+            }
         }
-        j.phase = 0;
-        j.phase_start = -1;  // uploads above are synchronous; the worker zeroes state
+        j.lane = group_lane(j.g->N);
+        // an RTM shot with cached dataTrue starts at the fwd(VS) phase
+        j.phase = j.kind == kRtm2d && j.have_true ? 1 : 0;
     }
 
+    static int group_lane(int N) { return N & 1; }
+
+    // ---- worker
+    void loop() {
+        try {
+            SSSI_HIP(hipSetDevice(device));
+            while (true) {
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv_work.wait(lk, [&] { return stop || !ready.empty() || !active.empty(); });
+                    if (stop && active.empty()) break;
+                    while (!ready.empty()) {
+                        ServeJob *j = ready.front();
+                        ready.pop_front();
+                        j->state = kActive;
+                        active.push_back(j);
+                    }
+                }
+                const double t0 = now_s();
+                for (ServeJob *j : active)
+                    if (!j->started) start_phase(*j);
+                run_tick();
+                busy_s += now_s() - t0;
+                if (stop) continue;
+            }
+            for (auto &l : lanes) SSSI_HIP(hipStreamSynchronize(l));
+        } catch (const std::exception &e) {
+            std::lock_guard<std::mutex> lk(mu);
+            fatal = e.what();
+            for (auto &kv : jobs)
+                if (kv.second->state != kDone) {
+                    kv.second->state = kFailed;
+                    kv.second->error = fatal;
+                }
+            cv_done.notify_all();
+        }
+    }
+
+    Group &group_of(ServeJob &j) {
+        const auto key = std::make_tuple(j.g->N, j.g->rz, j.g->ng);
+        auto it = groups.find(key);
+        if (it == groups.end()) {
+            auto g = std::make_unique<Group>();
+            g->N = j.g->N; g->rz = j.g->rz; g->ng = j.g->ng; g->lane = group_lane(j.g->N);
+            it = groups.emplace(key, std::move(g)).first;
+        }
+        return *it->second;
+    }
+
+    // descriptor of the job's current phase, starting at its group's next step
     void start_phase(ServeJob &j) {
-        j.phase_start = tick;
-        if (j.kind == kFwd3d) return;
+        j.started = true;
+        j.steps = 0;
+        hipStream_t s = lanes[j.lane];
+        if (j.kind == kFwd3d) {
+            j.p3->use_stream(s);  // state zeroed at construction
+            return;
+        }
         Job2D J;
         if (j.kind == kFwd2d || (j.kind == kRtm2d && j.phase == 0)) {
-            j.g->zero_state(stream);
+            j.g->zero_state(s);
             J = j.g->job();
             J.reverse = 0;
             J.src_ptr = j.src.ptr.p; J.src_cell = j.src.cell.p; J.src_amp = j.src.amp.p;
             J.nsrc = j.src.n; J.src_nt = j.src.nt;
             J.rec = j.rec.p;
         } else if (j.phase == 1) {
-            j.gs->zero_state(stream);
-            SSSI_HIP(hipMemsetAsync(j.ckmax.p, 0, j.ckmax.bytes(), stream));
+            j.gs->zero_state(s);
+            SSSI_HIP(hipMemsetAsync(j.ckmax.p, 0, j.ckmax.bytes(), s));
             J = j.gs->job();
             J.reverse = 0;
             J.src_ptr = j.src_s.ptr.p; J.src_cell = j.src_s.cell.p; J.src_amp = j.src_s.amp.p;
@@ -347,9 +450,9 @@ class Server {
             J.ck32 = j.ck32.p; J.ck_stage = j.stage.p; J.ck16 = j.ck16.p;
             J.ck_max = j.ckmax.p; J.ck_scale = j.ckscale.p;
         } else {
-            j.g->zero_state(stream);
-            SSSI_HIP(hipMemsetAsync(j.num.p, 0, j.num.bytes(), stream));
-            SSSI_HIP(hipMemsetAsync(j.den.p, 0, j.den.bytes(), stream));
+            j.g->zero_state(s, false);  // keep the dataTrue phase's watchdog report
+            SSSI_HIP(hipMemsetAsync(j.num.p, 0, j.num.bytes(), s));
+            SSSI_HIP(hipMemsetAsync(j.den.p, 0, j.den.bytes(), s));
             J = j.g->job();
             J.reverse = 1;
             J.surf = j.rec.p; J.surf2 = j.rec2.p; J.surf_nt = (int)j.nt;
@@ -357,146 +460,163 @@ class Server {
             J.im_ck32 = j.ck32.p; J.im_ck16 = j.ck16.p; J.im_scale = j.ckscale.p;
             J.im_num = j.num.p; J.im_den = j.den.p;
         }
-        J.start_step = (int)tick;
+        Group &G = group_of(j);
+        J.start_step = G.step;
         J.nt = (int)j.nt;
         j.job = J;
-        groups_dirty = true;
+        G.dirty = true;
     }
 
-    void rebuild_groups() {
-        for (auto &g : groups) retired.push_back(std::move(g));
-        groups.clear();
-        std::map<std::tuple<int, int, int>, Group *> by;
-        for (ServeJob *j : active) {
-            if (j->kind == kFwd3d || j->state != kActive) continue;
-            auto key = std::make_tuple(j->g->N, j->g->rz, j->g->ng);
-            if (!by.count(key)) {
-                groups.emplace_back(new Group{std::get<0>(key), std::get<1>(key),
-                                              std::get<2>(key), {}, {}, {}, 0});
-                by[key] = groups.back().get();
-            }
-            by[key]->members.push_back(j);
-        }
-        for (auto &g : groups) {
-            std::vector<const Cpml2dGrid *> grids;
-            std::vector<Job2D> desc;
-            for (ServeJob *j : g->members) {
-                const Cpml2dGrid *gg = (j->kind == kRtm2d && j->phase == 1) ? j->gs.get()
-                                                                            : j->g.get();
-                grids.push_back(gg);
-                desc.push_back(j->job);
-            }
-            std::vector<int2> ord;
-            build_block_order(grids, ord);
-            g->nblk = (int)ord.size();
-            g->desc.alloc(desc.size());
-            g->order.alloc(ord.size());
-            // pageable source: the copy is staged before the call returns
-            SSSI_HIP(hipMemcpyAsync(g->desc.p, desc.data(), g->desc.bytes(), hipMemcpyHostToDevice,
-                                    stream));
-            SSSI_HIP(hipMemcpyAsync(g->order.p, ord.data(), g->order.bytes(),
-                                    hipMemcpyHostToDevice, stream));
-        }
-        groups_dirty = false;
-    }
-
-    void step_all() {
-        if (groups_dirty) rebuild_groups();
-        for (auto &g : groups) {
-            launch_cpml2d_batch(g->N, g->rz, g->ng, g->desc.p, g->order.p, g->nblk,
-                                (int)tick, stream);
-            launches++;
-            for (ServeJob *j : g->members)
-                cell_updates += (double)j->nz * j->nx;
-        }
+    void rebuild(Group &G) {
+        G.members.clear();
         for (ServeJob *j : active)
-            if (j->kind == kFwd3d && j->state == kActive) {
-                j->p3->run(tick - j->phase_start, 1, false, true, nullptr, nullptr);
-                launches++;
-                cell_updates += (double)j->nz * j->nx * j->ny;
-            }
-        ++tick;
-        // bound how far the host runs ahead of the device
-        if (tick % 64 == 0) {
-            SSSI_HIP(hipStreamSynchronize(stream));
-            retired.clear();
+            if (j->kind != kFwd3d && j->state == kActive && j->started &&
+                std::make_tuple(j->g->N, j->g->rz, j->g->ng) ==
+                    std::make_tuple(G.N, G.rz, G.ng))
+                G.members.push_back(j);
+        G.dirty = false;
+        if (G.members.empty()) return;
+        std::vector<const Cpml2dGrid *> grids;
+        std::vector<Job2D> desc;
+        for (ServeJob *j : G.members) {
+            grids.push_back((j->kind == kRtm2d && j->phase == 1) ? j->gs.get() : j->g.get());
+            desc.push_back(j->job);
         }
+        std::vector<int2> ord;
+        build_block_order(grids, ord);
+        G.nblk = (int)ord.size();
+        // reallocated on the lane's stream: the old buffers are freed in stream
+        // order, after the launches that still read them
+        hipStream_t s = lanes[G.lane];
+        alloc_stream() = s;
+        G.desc.alloc(desc.size());
+        G.order.alloc(ord.size());
+        alloc_stream() = nullptr;
+        // pageable sources: the copies are staged before the calls return
+        SSSI_HIP(hipMemcpyAsync(G.desc.p, desc.data(), G.desc.bytes(), hipMemcpyHostToDevice, s));
+        SSSI_HIP(hipMemcpyAsync(G.order.p, ord.data(), G.order.bytes(), hipMemcpyHostToDevice, s));
     }
 
-    void finish_phases() {
-        bool changed = false;
-        for (ServeJob *j : active) {
-            if (j->state != kActive || tick - j->phase_start < j->nt) continue;
-            if (j->kind == kRtm2d && j->phase < 2) {
-                if (j->phase == 1 && j->ck_fmt != kCkF32)
-                    launch_ck_pack(j->job, (int)((j->nt - 1) / j->img_stride), stream);
-                j->phase++;
-                start_phase(*j);
-                changed = true;
-                continue;
-            }
-            // results out
-            size_t n;
-            const float *src;
-            if (j->kind == kRtm2d) {
-                n = (size_t)j->nz * j->nx;
-                hipLaunchKernelGGL(serve_image, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                                   stream, j->num.p, j->den.p, j->eps, n, j->img.p);
-                SSSI_HIP(hipGetLastError());
-                src = j->img.p;
-            } else if (j->kind == kFwd2d) {
-                n = (size_t)j->nt * j->nx;
-                src = j->rec.p;
-            } else {
-                n = (size_t)j->nt * j->nx * j->ny;
-                src = j->p3->rec.p;
-            }
-            j->out.resize(n);
-            SSSI_HIP(hipMemcpyAsync(j->out.data(), src, n * sizeof(float), hipMemcpyDeviceToHost,
-                                    stream));
-            // watchdog words of the job's grids (every phase reports into them)
-            const int *hw[2] = {j->kind == kFwd3d ? j->p3->health_word() : j->g->health.p,
-                                j->gs ? j->gs->health.p : nullptr};
-            for (int q = 0; q < 2; ++q)
-                if (hw[q])
-                    SSSI_HIP(hipMemcpyAsync(&j->health_h[q], hw[q], sizeof(int),
-                                            hipMemcpyDeviceToHost, stream));
-            SSSI_HIP(hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming));
-            SSSI_HIP(hipEventRecord(j->done_ev, stream));
-            j->state = kDraining;
-            changed = true;
+    hipEvent_t take_event() {
+        if (!spare_events.empty()) {
+            hipEvent_t e = spare_events.back();
+            spare_events.pop_back();
+            return e;
         }
-        if (changed) groups_dirty = true;
-        // completed jobs (results on the host): the helper releases their memory
+        hipEvent_t e;
+        SSSI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return e;
+    }
+
+    // a job's GPU work is all issued: its result is ready when done_ev is
+    void end_job(ServeJob &j) {
+        hipStream_t s = lanes[j.lane];
+        if (j.kind == kRtm2d) {
+            const size_t n = (size_t)j.nz * j.nx;
+            hipLaunchKernelGGL(serve_image, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               j.num.p, j.den.p, j.eps, n, j.img.p);
+            SSSI_HIP(hipGetLastError());
+            j.res_dev = j.img.p;
+            j.res_n = n;
+        } else if (j.kind == kFwd2d) {
+            j.res_dev = j.rec.p;
+            j.res_n = (size_t)j.nt * j.nx;
+        } else {
+            j.res_dev = j.p3->rec.p;
+            j.res_n = (size_t)j.nt * j.nx * j.ny;
+        }
+        SSSI_HIP(hipEventCreateWithFlags(&j.done_ev, hipEventDisableTiming));
+        SSSI_HIP(hipEventRecord(j.done_ev, s));
+        j.state = kDraining;
+    }
+
+    void run_tick() {
+        bool used_lane[kLanes] = {false, false, false, false};
+        // 2-D layouts: a chunk of batched steps, cut where a member's phase ends
+        for (auto &kv : groups) {
+            Group &G = *kv.second;
+            if (G.dirty) rebuild(G);
+            if (G.members.empty()) continue;
+            int64_t k = kChunk2;
+            for (ServeJob *j : G.members) k = std::min<int64_t>(k, j->nt - j->steps);
+            hipStream_t s = lanes[G.lane];
+            for (int64_t q = 0; q < k; ++q) {
+                launch_cpml2d_batch(G.N, G.rz, G.ng, G.desc.p, G.order.p, G.nblk, G.step, s);
+                ++G.step;
+                ++launches;
+            }
+            used_lane[G.lane] = true;
+            for (ServeJob *j : G.members) {
+                j->steps += k;
+                cell_updates += (double)k * j->nz * j->nx;
+                if (j->steps < j->nt) continue;
+                G.dirty = true;
+                if (j->kind == kRtm2d && j->phase < 2) {
+                    if (j->phase == 1) launch_ck_pack(j->job, (int)((j->nt - 1) / j->img_stride), s);
+                    j->phase++;
+                    start_phase(*j);
+                } else {
+                    end_job(*j);
+                }
+            }
+        }
+        // 3-D jobs: a chunk of steps each on the 3-D lanes
+        for (ServeJob *j : active) {
+            if (j->kind != kFwd3d || j->state != kActive) continue;
+            const int64_t k = std::min<int64_t>(kChunk3, j->nt - j->steps);
+            j->p3->run(j->steps, k, false, true, nullptr, nullptr);
+            j->steps += k;
+            launches += k;
+            cell_updates += (double)k * j->nz * j->nx * j->ny;
+            used_lane[j->lane] = true;
+            if (j->steps >= j->nt) {
+                j->p3->scan_field();
+                end_job(*j);
+            }
+        }
+        ++tick;
+        // bound the work in flight to kAhead ticks
+        std::vector<hipEvent_t> ev;
+        for (int l = 0; l < kLanes; ++l)
+            if (used_lane[l]) {
+                hipEvent_t e = take_event();
+                SSSI_HIP(hipEventRecord(e, lanes[l]));
+                ev.push_back(e);
+            }
+        inflight.push_back(std::move(ev));
+        while ((int)inflight.size() > kAhead || (!inflight.empty() && !any_running())) {
+            for (hipEvent_t e : inflight.front()) {
+                SSSI_HIP(hipEventSynchronize(e));
+                spare_events.push_back(e);
+            }
+            inflight.pop_front();
+        }
+        // jobs whose GPU work has completed go to the helpers
         std::vector<ServeJob *> keep;
+        bool changed = false;
         for (ServeJob *j : active) {
             if (j->state == kDraining && hipEventQuery(j->done_ev) == hipSuccess) {
                 std::lock_guard<std::mutex> lk(mu);
-                j->t_done = now_s();  // result is on the host
-                if (j->health_h[0] != kHealthy || j->health_h[1] != kHealthy) {
-                    const int h = std::min(j->health_h[0], j->health_h[1]);
-                    char msg[256];
-                    snprintf(msg, sizeof(msg),
-                             "non-finite wavefield (NaN/Inf) at tick %d: the time step is likely "
-                             "beyond the scheme's stability limit", h);
-                    j->error = msg;
-                    j->code = SSSI_EUNSTABLE;
-                }
                 finished.push_back(j);
                 cv_helper.notify_all();
+                changed = true;
                 continue;
             }
             keep.push_back(j);
         }
-        if (keep.size() != active.size()) groups_dirty = true;
         active.swap(keep);
-        // draining jobs only: wait for their copies instead of spinning
-        bool any_running = false;
-        for (ServeJob *j : active) any_running |= (j->state == kActive);
-        if (!any_running && !active.empty()) {
-            SSSI_HIP(hipStreamSynchronize(stream));
-            retired.clear();
+        if (changed)
+            for (auto &kv : groups) kv.second->dirty = true;
+        // only draining jobs left: wait for them instead of spinning
+        if (!any_running() && !active.empty()) {
+            for (auto &l : lanes) SSSI_HIP(hipStreamSynchronize(l));
         }
+    }
+
+    bool any_running() const {
+        for (ServeJob *j : active)
+            if (j->state == kActive) return true;
+        return false;
     }
 
     ServeJob &get(int64_t id) {
@@ -519,6 +639,7 @@ class Server {
     void drain() {
         std::unique_lock<std::mutex> lk(mu);
         cv_done.wait(lk, [&] {
+            if (!fatal.empty()) return true;
             for (auto &kv : jobs)
                 if (kv.second->state != kDone && kv.second->state != kFailed) return false;
             return true;
@@ -536,6 +657,17 @@ struct sssi_server {
 };
 
 namespace {
+// Copy of a caller's array into a job (inputs are copied at submission), split
+// over host threads for the large ones.
+template <class T, class U>
+void copy_in(std::vector<T> &dst, const U *src, size_t n) {
+    dst.resize(n);
+    T *d = dst.data();
+    parallel_ranges((int64_t)n, [&](int64_t a, int64_t b) {
+        for (int64_t q = a; q < b; ++q) d[q] = (T)src[q];
+    }, 1 << 20);
+}
+
 template <class F>
 int serve_guard(F &&f) {
     try {
@@ -587,7 +719,7 @@ int sssi_server_submit_fwd2d(sssi_server *s, const double *v, int64_t nz, int64_
         check_src(nx, src_ix, "x");
         auto j = std::make_unique<ServeJob>();
         j->kind = kFwd2d;
-        j->v.assign(v, v + nz * nx);
+        copy_in(j->v, v, (size_t)(nz * nx));
         j->wavelet.assign(wavelet, wavelet + nt);
         j->nz = nz; j->nx = nx; j->nt = nt; j->N = nDiffOrder; j->nB = nBoundary;
         j->dz = dz; j->dx = dx; j->dt = dt; j->src_iz = src_iz; j->src_ix = src_ix;
@@ -596,10 +728,11 @@ int sssi_server_submit_fwd2d(sssi_server *s, const double *v, int64_t nz, int64_
     });
 }
 
-int sssi_server_submit_rtm2d(sssi_server *s, const double *V, const double *VS, int64_t nz,
-                             int64_t nx, int64_t nt, int nDiffOrder, int nBoundary, double dz,
-                             double dx, double dt, int src_ix, const double *wavelet,
-                             int img_stride, int ck_format, double eps, int64_t *job_id) {
+int sssi_server_submit_rtm2d_data(sssi_server *s, const double *V, const double *VS, int64_t nz,
+                                  int64_t nx, int64_t nt, int nDiffOrder, int nBoundary,
+                                  double dz, double dx, double dt, int src_ix,
+                                  const double *wavelet, const double *data_true, int img_stride,
+                                  int ck_format, double eps, int64_t *job_id) {
     return serve_guard([&] {
         if (!V || !VS || !wavelet) fail(SSSI_EINVAL, "V, VS and wavelet must not be NULL");
         validate_common(2, nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
@@ -609,17 +742,32 @@ int sssi_server_submit_rtm2d(sssi_server *s, const double *V, const double *VS, 
         if (img_stride < 1) fail(SSSI_EINVAL, "img_stride must be >= 1");
         if (ck_format < SSSI_CK_F32 || ck_format > SSSI_CK_BF16)
             fail(SSSI_EINVAL, "unknown checkpoint format %d", ck_format);
+        if (!(eps >= 0) || !std::isfinite(eps)) fail(SSSI_EINVAL, "eps must be finite, >= 0");
         auto j = std::make_unique<ServeJob>();
         j->kind = kRtm2d;
-        j->v.assign(V, V + nz * nx);
-        j->vs.assign(VS, VS + nz * nx);
+        copy_in(j->v, V, (size_t)(nz * nx));
+        copy_in(j->vs, VS, (size_t)(nz * nx));
         j->wavelet.assign(wavelet, wavelet + nt);
+        if (data_true) {
+            // data(nx, nt) column-major is [t][j]: rounded once to the device's FP32
+            copy_in(j->dtrue, data_true, (size_t)(nt * nx));
+            j->have_true = true;
+        }
         j->nz = nz; j->nx = nx; j->nt = nt; j->N = nDiffOrder; j->nB = nBoundary;
         j->dz = dz; j->dx = dx; j->dt = dt; j->src_iz = 0; j->src_ix = src_ix;
         j->img_stride = img_stride; j->ck_fmt = ck_format; j->eps = eps;
         const int64_t id = S(s).submit(std::move(j));
         if (job_id) *job_id = id;
     });
+}
+
+int sssi_server_submit_rtm2d(sssi_server *s, const double *V, const double *VS, int64_t nz,
+                             int64_t nx, int64_t nt, int nDiffOrder, int nBoundary, double dz,
+                             double dx, double dt, int src_ix, const double *wavelet,
+                             int img_stride, int ck_format, double eps, int64_t *job_id) {
+    return sssi_server_submit_rtm2d_data(s, V, VS, nz, nx, nt, nDiffOrder, nBoundary, dz, dx, dt,
+                                         src_ix, wavelet, nullptr, img_stride, ck_format, eps,
+                                         job_id);
 }
 
 int sssi_server_submit_fwd3d(sssi_server *s, const double *v, int64_t nz, int64_t nx, int64_t ny,
@@ -635,7 +783,7 @@ int sssi_server_submit_fwd3d(sssi_server *s, const double *v, int64_t nz, int64_
         check_src(ny, src_iy, "y");
         auto j = std::make_unique<ServeJob>();
         j->kind = kFwd3d;
-        j->v.assign(v, v + nz * nx * ny);
+        copy_in(j->v, v, (size_t)(nz * nx * ny));
         j->wavelet.assign(wavelet, wavelet + nt);
         j->nz = nz; j->nx = nx; j->ny = ny; j->nt = nt; j->N = nDiffOrder; j->nB = nBoundary;
         j->dz = dz; j->dx = dx; j->dy = dy; j->dt = dt;

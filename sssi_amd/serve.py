@@ -48,14 +48,21 @@ class Server:
         return jid.value
 
     def submit_rtm2d(self, V, VS, nt, nDiffOrder, nBoundary, dz, dx, dt, src_ix, wavelet,
-                     img_stride=1, ck_format="fp32", eps=1e-3):
+                     img_stride=1, ck_format="fp32", eps=1e-3, data_true=None):
+        """One RTM shot on the extended grid of V; with data_true (nx, nt), the
+        cached observed data, the job skips its fwd(V) propagation."""
         V = _f64(V)
         VS = _f64(VS)
         w = np.ascontiguousarray(wavelet, dtype=np.float64)
+        dtr = None
+        if data_true is not None:
+            dtr = _f64(data_true)
+            if dtr.shape != (V.shape[1], int(nt)):
+                raise ValueError("data_true must be (nx, nt)")
         jid = ctypes.c_int64(0)
-        _call("sssi_server_submit_rtm2d", self._h, _dp(V), _dp(VS), V.shape[0], V.shape[1],
+        _call("sssi_server_submit_rtm2d_data", self._h, _dp(V), _dp(VS), V.shape[0], V.shape[1],
               int(nt), int(nDiffOrder), int(nBoundary), float(dz), float(dx), float(dt),
-              int(src_ix), _dp(w), int(img_stride), CK_FORMATS[ck_format], float(eps),
+              int(src_ix), _dp(w), _dp(dtr), int(img_stride), CK_FORMATS[ck_format], float(eps),
               ctypes.byref(jid))
         self._shapes[jid.value] = V.shape
         return jid.value

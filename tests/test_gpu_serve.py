@@ -118,3 +118,26 @@ def test_oversized_job_is_rejected():
         with pytest.raises(sssi_amd.SssiError) as e:
             submit(s, kind, kw)
         assert e.value.code == -5
+
+
+def test_served_rtm_with_cached_data_true_equals_computing_it():
+    """A served RTM shot given its cached dataTrue (as the reference's apps load
+    it instead of recomputing, mainRtmTimeCpmlFor2dAw.m:170-184) skips the
+    fwd(V) propagation and produces the same image bit for bit."""
+    kind, kw = make_jobs()[2]
+    nB = kw["nBoundary"]
+    V, VS = kw["V"], kw["VS"]
+    pz, px = V.shape[0] - nB, V.shape[1] - 2 * nB
+    r = sssi_amd.Rtm2D(V[:pz, nB:nB + px], VS[:pz, nB:nB + px], kw["nDiffOrder"], nB, kw["dz"],
+                       kw["dx"], kw["dt"], kw["nt"], img_stride=kw["img_stride"],
+                       ck_format=kw["ck_format"])
+    r.add_shot(0, px, kw["src_ix"] - nB, kw["wavelet"])
+    r.precompute_data()
+    dtrue = r.shot(0)["data_true"]
+    r.close()
+    with sssi_amd.Server(hbm_budget_bytes=8e9, max_active=4) as s:
+        a = s.submit_rtm2d(**kw)
+        b = s.submit_rtm2d(**kw, data_true=dtrue)
+        s.drain()
+        ia, ib = s.result(a), s.result(b)
+    assert np.any(ia != 0) and np.array_equal(ia, ib)
