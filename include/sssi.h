@@ -246,8 +246,40 @@ int sssi_rtm2d_get_stack(sssi_rtm *r, double *stack);
  * (image, M, s2 without eps) and dataTrue (nx_ext, nt); any may be NULL. */
 int sssi_rtm2d_get_shot(sssi_rtm *r, int shot, double *image, double *num, double *den,
                         double *data_true);
+/* Durable per-shot results with resume by existence, as the reference's RTM
+ * apps keep them (applications/rtm_2d_acoustic/mainRtmTimeCpmlFor2dAw.m:170-184,
+ * :259-264; mainRtmTimeSpongeFor2dAw.m:165-179, :255-259, :343-346): with a
+ * cache directory set (created if missing; NULL or "" turns caching off),
+ * dataTrue<xs>.mat (variable dataTrue, (nx_ext, nt)) and image<xs>.mat
+ * (variable image, the shot's image on its extended window) are loaded instead
+ * of recomputed when they exist, and written when they do not; after a
+ * migration StackedRTM.mat (variable StackedRTM, (nz, nx)) is written if
+ * absent.  xs = x_begin + src_ix + 1 is the shot's source column (1-based),
+ * the reference's file key.  Files are Level 5 MAT-files (MATLAB `load`
+ * reads them; the reference's -v7.3 needs HDF5, absent here), each with an
+ * sssiShot variable of the parameters and input checksums it was made from; a
+ * file made from other inputs fails the call (SSSI_ESTATE).  A resumed
+ * migration equals an uninterrupted one bit for bit. */
+int sssi_rtm2d_set_cache(sssi_rtm *r, const char *dir);
+/* Shots whose dataTrue / image came from the cache in the last precompute /
+ * migration, and files written by the handle so far. */
+int sssi_rtm2d_cache_stats(sssi_rtm *r, int *loaded_data, int *loaded_images, int *saved);
 /* Device memory held by the handle. */
 int sssi_rtm2d_device_bytes(sssi_rtm *r, double *bytes);
+
+/* ---------------------------------------------------------------------
+ * MAT-file (Level 5) I/O of real double arrays: the durable format of the
+ * results above, for callers without MATLAB.  sssi_mat_write writes nvar
+ * variables (names[v], data[v] column-major, ndims[v] dimensions each, dims
+ * concatenated) to a temporary file renamed over path.  sssi_mat_read finds
+ * variable `name` and returns its dimensions (up to 8) and, with out != NULL,
+ * its n values (any numeric storage type, zlib-compressed elements included).
+ * No counterpart in the reference, whose apps call MATLAB's save/load.
+ * ------------------------------------------------------------------- */
+int sssi_mat_write(const char *path, int nvar, const char *const *names,
+                   const double *const *data, const int *ndims, const int64_t *dims);
+int sssi_mat_read(const char *path, const char *name, int *ndims, int64_t *dims, double *out,
+                  int64_t n);
 void sssi_rtm2d_destroy(sssi_rtm *r);
 
 /* ---------------------------------------------------------------------

@@ -65,7 +65,12 @@ SIGNATURES = {
     "sssi_rtm2d_migrate": [_vp, _int, _D, ctypes.POINTER(_i64)],
     "sssi_rtm2d_get_stack": [_vp, _D],
     "sssi_rtm2d_get_shot": [_vp, _int, _D, _D, _D, _D],
+    "sssi_rtm2d_set_cache": [_vp, ctypes.c_char_p],
+    "sssi_rtm2d_cache_stats": [_vp, _I, _I, _I],
     "sssi_rtm2d_device_bytes": [_vp, _D],
+    "sssi_mat_write": [ctypes.c_char_p, _int, ctypes.POINTER(ctypes.c_char_p),
+                       ctypes.POINTER(_D), _I, ctypes.POINTER(_i64)],
+    "sssi_mat_read": [ctypes.c_char_p, ctypes.c_char_p, _I, ctypes.POINTER(_i64), _D, _i64],
     "sssi_rtm2d_destroy": [_vp],
     "sssi_server_create": [_dbl, _int, ctypes.POINTER(_vp)],
     "sssi_server_submit_fwd2d": [_vp, _D, _i64, _i64, _i64, _int, _int, _dbl, _dbl, _dbl, _int,

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsssi.so")
 
 SOURCES = ["common.cpp", "tune.cpp", "abi.cpp", "cpml2d.hip", "prop2d.hip", "prop3d.hip", "rtm.hip",
-           "serve.cpp", "noalloc.hip"]
+           "serve.cpp", "noalloc.hip", "matfile.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -48,7 +48,7 @@ def build(verbose=False, jobs=8):
     for p in procs:
         _wait(p)
     tmp = LIB + ".tmp"
-    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread", "-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

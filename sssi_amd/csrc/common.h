@@ -163,6 +163,16 @@ float cpml2d_bm1_z(const double *v, int64_t nz, int64_t nx, int nB, double dz, d
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
+// Level 5 MAT-files of real double arrays (matfile.cpp).  mat_write writes
+// nvar variables (dims concatenated, ndims[v] each) atomically (temporary file
+// renamed over the target); mat_read returns false if the file holds no such
+// variable, and fills ndims/dims (up to 8) and, when out is given, n values.
+void mat_write(const char *path, int nvar, const char *const *names, const double *const *data,
+               const int *ndims, const int64_t *dims);
+bool mat_read(const char *path, const char *name, int *ndims, int64_t *dims, double *out,
+              int64_t n);
+bool file_exists(const char *path);
+
 // Argument checks shared by every entry point (before any device work).
 // dim 3 enforces 2*nB <= ny also for ny = 1 (a 3-D call with a 2-D model).
 void validate_common(int dim, int64_t nz, int64_t nx, int64_t ny, int64_t nt, int N, int nB,

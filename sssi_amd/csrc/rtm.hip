@@ -13,10 +13,13 @@
 // Two execution modes over the same kernels: one shot per launch (each time
 // step of each shot is its own launch) or batched (one launch per time step
 // covers the tiles of every shot, strip tiles first).
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <string>
 
 #include "cpml2d.h"
 
@@ -37,6 +40,17 @@ __global__ void rtm_stack_image(const float *num, const float *den, double eps, 
         stack[(size_t)(x_begin + jj) * nz_stack + i] += (double)img;  // column-major stack
 }
 
+// A cached per-shot image (FP32 values, as computed) into the stack.
+__global__ void rtm_stack_cached(const float *img, int nz_ext, int nx_ext, int nz_phys, int nB,
+                                 int x_begin, int x_count, double *stack, int nz_stack) {
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= nz_ext || j >= nx_ext) return;
+    const int jj = j - nB;
+    if (i < nz_phys && jj >= 0 && jj < x_count)
+        stack[(size_t)(x_begin + jj) * nz_stack + i] += (double)img[(size_t)i * nx_ext + j];
+}
+
 __global__ void fill_f32(float *p, size_t n, float v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -55,7 +69,19 @@ struct RtmShot {
     DevBuf<float> ckscale;
     DevBuf<float> num, den, img;
     bool have_true = false;
+    bool img_cached = false;  // image loaded from the result cache (not migrated)
+    uint64_t hV = 0, hVS = 0, hW = 0;  // checksums of the window models and wavelet
 };
+
+// FNV-1a over raw bytes: identifies the inputs a cached result was made from.
+static uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603ull) {
+    const uint8_t *b = (const uint8_t *)p;
+    for (size_t q = 0; q < n; ++q) {
+        h ^= b[q];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
 
 class Rtm2D {
    public:
@@ -67,6 +93,9 @@ class Rtm2D {
     DevBuf<double> stack;       // nz x nx physical, column-major
     hipStream_t stream = nullptr;
     int64_t launches = 0;
+    // per-shot result cache (MAT-files in cache_dir, empty: off)
+    std::string cache_dir;
+    int loaded_true = 0, loaded_img = 0, saved_files = 0;
 
     Rtm2D(const double *V_, const double *VS_, int64_t nz_, int64_t nx_, int N_, int nB_,
           double dz_, double dx_, double dt_, int64_t nt_, int stride_, int fmt_, double eps_)
@@ -114,6 +143,9 @@ class Rtm2D {
         std::vector<double> ws = window_ext(VS, x_begin, x_count);
         s->gV.reset(new Cpml2dGrid(wv.data(), ze, xe, N, nB, dz, dx, dt));
         s->gS.reset(new Cpml2dGrid(ws.data(), ze, xe, N, nB, dz, dx, dt));
+        s->hV = fnv1a(wv.data(), wv.size() * sizeof(double));
+        s->hVS = fnv1a(ws.data(), ws.size() * sizeof(double));
+        s->hW = fnv1a(s->wavelet.data(), s->wavelet.size() * sizeof(double));
         const int iz = 0, ix = src_ix + nB;
         s->src.set(*s->gS, 1, &iz, &ix, s->wavelet.data(), nt);
         const size_t ncell = (size_t)ze * xe;
@@ -232,7 +264,106 @@ class Rtm2D {
         }
     }
 
+    // ---- per-shot result cache.  The reference's apps name their files by the
+    // shot's source column xs (1-based) in the model: dataTrue<xs>.mat holds
+    // dataTrue (nx_ext x nt), image<xs>.mat the shot's image M./(eps+s2) on its
+    // extended window, StackedRTM.mat the stack; a file is written only if it
+    // does not exist yet (mainRtmTimeCpmlFor2dAw.m:170-184, :259-264,
+    // mainRtmTimeSpongeFor2dAw.m:343-346).  Each file also holds sssiShot, the
+    // parameters and input checksums it was made from; a file made for other
+    // inputs is refused rather than used.
+    int xs_of(const RtmShot &s) const { return (int)(s.x_begin + s.src_ix + 1); }
+    std::string cache_path(const char *what, const RtmShot &s) const {
+        return cache_dir + "/" + what + std::to_string(xs_of(s)) + ".mat";
+    }
+    std::vector<double> key_of(const RtmShot &s, bool image) const {
+        auto split = [](uint64_t h, std::vector<double> &k) {
+            k.push_back((double)(h >> 32));
+            k.push_back((double)(h & 0xffffffffu));
+        };
+        std::vector<double> k = {(double)s.x_begin, (double)s.x_count, (double)s.src_ix,
+                                 (double)nz, (double)nt, (double)N, (double)nB, dz, dx, dt};
+        split(s.hV, k);
+        split(s.hW, k);
+        if (image) {
+            k.push_back((double)stride);
+            k.push_back((double)fmt);
+            k.push_back(eps);
+            split(s.hVS, k);
+        }
+        return k;
+    }
+    // the variable `name` of a cached file if it exists and was made from the
+    // same inputs (n values)
+    bool cache_load(const std::string &path, const char *name, const std::vector<double> &key,
+                    std::vector<double> &val, size_t n) {
+        if (cache_dir.empty() || !file_exists(path.c_str())) return false;
+        int nd = 0;
+        int64_t dims[8] = {0};
+        std::vector<double> k;
+        bool same = mat_read(path.c_str(), "sssiShot", &nd, dims, nullptr, 0) && nd == 2 &&
+                    (size_t)(dims[0] * dims[1]) == key.size();
+        if (same) {
+            k.resize(key.size());
+            mat_read(path.c_str(), "sssiShot", nullptr, nullptr, k.data(), (int64_t)k.size());
+            same = k == key;
+        }
+        if (!same)
+            fail(SSSI_ESTATE, "cache file %s was made for other inputs (shot parameters or "
+                 "models differ): remove it or use another cache directory", path.c_str());
+        val.resize(n);
+        if (!mat_read(path.c_str(), name, &nd, dims, nullptr, 0) ||
+            (size_t)(nd == 2 ? dims[0] * dims[1] : 0) != n)
+            fail(SSSI_ESTATE, "cache file %s holds no %s of %zu values", path.c_str(), name, n);
+        mat_read(path.c_str(), name, nullptr, nullptr, val.data(), (int64_t)n);
+        return true;
+    }
+    void cache_save(const std::string &path, const char *name, const double *val, int64_t d0,
+                    int64_t d1, const std::vector<double> &key) {
+        if (cache_dir.empty() || file_exists(path.c_str())) return;
+        const char *names[2] = {name, "sssiShot"};
+        const double *data[2] = {val, key.data()};
+        const int nd[2] = {2, 2};
+        const int64_t dims[4] = {d0, d1, 1, (int64_t)key.size()};
+        mat_write(path.c_str(), 2, names, data, nd, dims);
+        ++saved_files;
+    }
+    void check_cache_names() const {
+        for (size_t a = 0; a < shots.size(); ++a)
+            for (size_t b = a + 1; b < shots.size(); ++b)
+                if (xs_of(*shots[a]) == xs_of(*shots[b]))
+                    fail(SSSI_EINVAL, "shots %zu and %zu share source column %d, which names "
+                         "their cache files", a, b, xs_of(*shots[a]));
+    }
+    // device FP32 array (n values) <-> host doubles
+    void to_host(const float *d, size_t n, std::vector<double> &out) {
+        std::vector<float> h(n);
+        SSSI_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+        SSSI_HIP(hipStreamSynchronize(stream));
+        out.assign(h.begin(), h.end());
+    }
+    void to_device(const std::vector<double> &v, float *d) {
+        std::vector<float> h(v.begin(), v.end());  // values are FP32 results: exact
+        SSSI_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,
+                                stream));
+        SSSI_HIP(hipStreamSynchronize(stream));
+    }
+
     void precompute_true(bool batched) {
+        loaded_true = 0;
+        if (!cache_dir.empty()) {
+            check_cache_names();
+            std::vector<double> val;
+            for (auto &s : shots) {
+                if (s->have_true) continue;
+                const size_t n = (size_t)nt * (s->x_count + 2 * nB);
+                if (cache_load(cache_path("dataTrue", *s), "dataTrue", key_of(*s, false), val, n)) {
+                    to_device(val, s->dtrue.p);  // data(nx_ext, nt) is [t][j]
+                    s->have_true = true;
+                    ++loaded_true;
+                }
+            }
+        }
         std::vector<RtmShot *> todo;
         for (auto &s : shots)
             if (!s->have_true) todo.push_back(s.get());
@@ -255,6 +386,15 @@ class Rtm2D {
         SSSI_HIP(hipStreamSynchronize(stream));
         for (auto *s : todo) check_shot(*s, *s->gV, "V");
         for (auto *s : todo) s->have_true = true;
+        if (!cache_dir.empty()) {
+            std::vector<double> val;
+            for (auto *s : todo) {
+                const int64_t xe = s->x_count + 2 * nB;
+                to_host(s->dtrue.p, (size_t)nt * xe, val);
+                cache_save(cache_path("dataTrue", *s), "dataTrue", val.data(), xe, nt,
+                           key_of(*s, false));
+            }
+        }
     }
 
     // watchdog of a shot's grid after its phase ran (the stream is idle)
@@ -271,42 +411,100 @@ class Rtm2D {
         }
     }
 
-    // the imaging job of every shot: fwd(VS) + checkpoints, rvs(V) + imaging, stack
+    // the imaging job of every shot: fwd(VS) + checkpoints, rvs(V) + imaging,
+    // stack (in shot order); shots with a cached image are stacked from it
     void migrate(bool batched) {
         SSSI_HIP(hipMemsetAsync(stack.p, 0, stack.bytes(), stream));
-        if (batched) {
+        loaded_img = 0;
+        std::vector<RtmShot *> run;
+        {
+            std::vector<double> val;
+            if (!cache_dir.empty()) check_cache_names();
+            for (auto &s : shots) {
+                const int64_t ze = nz + nB, xe = s->x_count + 2 * nB;
+                s->img_cached = false;
+                if (cache_load(cache_path("image", *s), "image", key_of(*s, true), val,
+                               (size_t)ze * xe)) {
+                    // column-major (ze, xe) -> the device's [i][j]
+                    std::vector<double> rows((size_t)ze * xe);
+                    for (int64_t j = 0; j < xe; ++j)
+                        for (int64_t i = 0; i < ze; ++i) rows[(size_t)i * xe + j] = val[j * ze + i];
+                    to_device(rows, s->img.p);
+                    s->img_cached = true;
+                    ++loaded_img;
+                } else {
+                    run.push_back(s.get());
+                }
+            }
+        }
+        if (batched && !run.empty()) {
             std::vector<Cpml2dGrid *> gs, gv;
             std::vector<Job2D> jf, jr;
-            for (auto &s : shots) {
+            for (RtmShot *s : run) {
                 prepare_fwd(*s);
                 gs.push_back(s->gS.get());
                 jf.push_back(job_fwd(*s));
             }
             run_batch_phase(gs, jf);
-            for (size_t k = 0; k < shots.size(); ++k) finish_fwd(*shots[k], jf[k]);
-            for (auto &s : shots) {
+            for (size_t k = 0; k < run.size(); ++k) finish_fwd(*run[k], jf[k]);
+            for (RtmShot *s : run) {
                 prepare_rvs(*s);
                 gv.push_back(s->gV.get());
                 jr.push_back(job_rvs(*s));
             }
             run_batch_phase(gv, jr);
-            for (auto &s : shots) finish_shot(*s);
         } else {
-            for (auto &s : shots) {
+            for (RtmShot *s : run) {
                 prepare_fwd(*s);
                 const Job2D jf = job_fwd(*s);
                 run_single_phase(*s->gS, jf);
                 finish_fwd(*s, jf);
                 prepare_rvs(*s);
                 run_single_phase(*s->gV, job_rvs(*s));
-                finish_shot(*s);
             }
         }
-        SSSI_HIP(hipStreamSynchronize(stream));
         for (auto &s : shots) {
+            if (s->img_cached) stack_cached(*s);
+            else finish_shot(*s);
+        }
+        SSSI_HIP(hipStreamSynchronize(stream));
+        for (RtmShot *s : run) {
             check_shot(*s, *s->gS, "VS");
             check_shot(*s, *s->gV, "V");
         }
+        if (!cache_dir.empty()) {
+            std::vector<double> val;
+            for (RtmShot *s : run) {
+                const int64_t ze = nz + nB, xe = s->x_count + 2 * nB;
+                to_host(s->img.p, (size_t)ze * xe, val);
+                std::vector<double> cm((size_t)ze * xe);
+                for (int64_t j = 0; j < xe; ++j)
+                    for (int64_t i = 0; i < ze; ++i) cm[j * ze + i] = val[(size_t)i * xe + j];
+                cache_save(cache_path("image", *s), "image", cm.data(), ze, xe, key_of(*s, true));
+            }
+            const std::string sp = cache_dir + "/StackedRTM.mat";
+            if (!file_exists(sp.c_str())) {
+                std::vector<double> st((size_t)nz * nx);
+                SSSI_HIP(hipMemcpyAsync(st.data(), stack.p, stack.bytes(), hipMemcpyDeviceToHost,
+                                        stream));
+                SSSI_HIP(hipStreamSynchronize(stream));
+                const char *names[1] = {"StackedRTM"};
+                const double *data[1] = {st.data()};
+                const int nd[1] = {2};
+                const int64_t dims[2] = {nz, nx};
+                mat_write(sp.c_str(), 1, names, data, nd, dims);
+                ++saved_files;
+            }
+        }
+    }
+
+    void stack_cached(RtmShot &s) {
+        const int ze = (int)(nz + nB), xe = (int)(s.x_count + 2 * nB);
+        dim3 grid((unsigned)((xe + 63) / 64), (unsigned)((ze + 3) / 4));
+        hipLaunchKernelGGL(rtm_stack_cached, grid, dim3(256), 0, stream, s.img.p, ze, xe, (int)nz,
+                           nB, (int)s.x_begin, (int)s.x_count, stack.p, (int)nz);
+        SSSI_HIP(hipGetLastError());
+        launches++;
     }
 };
 
@@ -436,6 +634,31 @@ int sssi_rtm2d_get_shot(sssi_rtm *r, int shot, double *image, double *num, doubl
             SSSI_HIP(hipStreamSynchronize(rt.stream));
             for (size_t q = 0; q < d.size(); ++q) data_true[q] = d[q];
         }
+    });
+}
+
+int sssi_rtm2d_set_cache(sssi_rtm *r, const char *dir) {
+    return rtm_guard([&] {
+        Rtm2D &rt = R(r);
+        if (!dir || !*dir) {
+            rt.cache_dir.clear();
+            return;
+        }
+        struct stat st;
+        if (stat(dir, &st) != 0 && mkdir(dir, 0755) != 0)
+            fail(SSSI_EINVAL, "cannot create cache directory %s", dir);
+        if (stat(dir, &st) != 0 || !S_ISDIR(st.st_mode))
+            fail(SSSI_EINVAL, "%s is not a directory", dir);
+        rt.cache_dir = dir;
+    });
+}
+
+int sssi_rtm2d_cache_stats(sssi_rtm *r, int *loaded_data, int *loaded_images, int *saved) {
+    return rtm_guard([&] {
+        Rtm2D &rt = R(r);
+        if (loaded_data) *loaded_data = rt.loaded_true;
+        if (loaded_images) *loaded_images = rt.loaded_img;
+        if (saved) *saved = rt.saved_files;
     });
 }
 

@@ -82,6 +82,21 @@ class Rtm2D:
         _call("sssi_rtm2d_get_shot", self._h, int(k), _dp(img), _dp(num), _dp(den), _dp(dtrue))
         return {"image": img, "num": num, "den": den, "data_true": dtrue}
 
+    def set_cache(self, directory):
+        """Durable per-shot results in `directory` (created if missing; None turns
+        caching off): dataTrue<xs>.mat and image<xs>.mat are loaded when they
+        exist and written when they do not, StackedRTM.mat after a migration
+        (xs = x_begin + src_ix + 1, the shot's 1-based source column)."""
+        _call("sssi_rtm2d_set_cache", self._h, None if directory is None else
+              str(directory).encode())
+
+    def cache_stats(self):
+        """(dataTrue loaded, images loaded) in the last run, files written so far."""
+        a, b, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _call("sssi_rtm2d_cache_stats", self._h, ctypes.byref(a), ctypes.byref(b),
+              ctypes.byref(c))
+        return {"loaded_data": a.value, "loaded_images": b.value, "saved": c.value}
+
     def device_bytes(self):
         b = ctypes.c_double(0)
         _call("sssi_rtm2d_device_bytes", self._h, ctypes.byref(b))
