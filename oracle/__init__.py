@@ -61,6 +61,11 @@ def lib():
                                            ctypes.c_double, ctypes.c_double, _i64,
                                            ctypes.c_double, _D, _D, _D, _D, _D]
         L.orc_xcorr_image.argtypes = [_D, _D, _i64, _i64, _i64, ctypes.c_double, _D, _D, _D]
+        L.orc_eikonal2d.argtypes = [_D, _i64, _i64, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, _D]
+        L.orc_kirchhoff_migrate.argtypes = [_D, _i64, _i64, _i64, _I, ctypes.c_int, _I,
+                                            ctypes.c_int, _D, _i64, ctypes.c_double,
+                                            ctypes.c_int, _D]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -246,3 +251,31 @@ def xcorr_image(fwd_snap, rvs_snap, stride=1, eps=1e-3):
     _check(lib().orc_xcorr_image(_dp(f), _dp(r), ncell, nt, stride, eps, _dp(num), _dp(den),
                                  _dp(img)), "xcorr_image")
     return num, den, img
+
+
+def eikonal2d(V, dx, sz, sx, iMax):
+    """T = eikonal2d(V, dx, sz, sx, iMax) (src/fd_cmex/eikonal2d_mex.cpp); sz, sx 1-based."""
+    V = _f64(V)
+    nz, nx = V.shape
+    T = np.zeros((nz, nx), order="F")
+    _check(lib().orc_eikonal2d(_dp(V), nz, nx, float(dx), int(sz), int(sx), int(iMax), _dp(T)),
+           "eikonal2d")
+    return T
+
+
+def kirchhoff_migrate(travelTime, xRecGrid, xShotAndRecGrid, shot, dt, nz, ixs):
+    """m = migrate(travelTime, xRecGrid, xShotAndRecGrid, shot, dt, nz, ixs)
+    (src/migrate.m); grids and ixs 1-based, shot (nt, nx)."""
+    tt = _f64(travelTime)
+    if tt.ndim == 2:
+        tt = tt[:, :, None]
+    nzt, nx, ntab = tt.shape
+    assert nzt == nz
+    shot = _f64(shot)
+    xr = np.ascontiguousarray(np.atleast_1d(xRecGrid), dtype=np.int32)
+    xs = np.ascontiguousarray(np.atleast_1d(xShotAndRecGrid), dtype=np.int32)
+    m = np.zeros((nz, nx), order="F")
+    _check(lib().orc_kirchhoff_migrate(_dp(tt), nz, nx, ntab, _ip(xr), xr.size, _ip(xs), xs.size,
+                                       _dp(shot), shot.shape[0], float(dt), int(ixs), _dp(m)),
+           "kirchhoff_migrate")
+    return m

@@ -806,3 +806,104 @@ done:
     free(dtrue); free(dsm); free(fsnap); free(rsnap);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Traveltimes: src/fd_cmex/eikonal2d_mex.cpp:27-267 (Zhao's fast sweeping as
+ * the reference codes it).  Every "sweep" reads only TOld and writes TNew, so
+ * each of the four sweeps of an iteration is one Jacobi pass over the grid
+ * (their loop orders do not change the result), followed by TNew(src) = 0 and
+ * TOld = TNew.  Per cell, with neighbours taken inside the grid only (a row or
+ * column at the edge uses its single inner neighbour):
+ *   a = min(T(i-1,j), T(i+1,j)),  b = min(T(i,j-1), T(i,j+1))
+ *   TT = |a-b| < s*dx ? 0.5*(a + b + sqrt(2*s*s*dx*dx - (a-b)*(a-b)))
+ *                     : min(a,b) + s*dx
+ *   TNew = min(TOld, TT)
+ * s = 1/v; dx is truncated to an integer (eikonal2d_mex.cpp:35, `(int)`);
+ * T starts at 65535 (:31) except T(src) = 0; sz, sx are 1-based.  The
+ * operation order above is the MEX's (built with -ffp-contract=off here). */
+int orc_eikonal2d(const double *v, int64_t nz, int64_t nx, double dx_in, int sz, int sx,
+                  int iMax, double *T)
+{
+    if (!v || !T || nz < 2 || nx < 2 || sz < 1 || sz > nz || sx < 1 || sx > nx || iMax < 0)
+        return ORC_EINVAL;
+    const int64_t nn = nz * nx;
+    const double dx = (double)(int)dx_in;
+    double *slow = malloc(nn * sizeof(double));
+    double *told = malloc(nn * sizeof(double));
+    if (!slow || !told) { free(slow); free(told); return ORC_ENOMEM; }
+    for (int64_t q = 0; q < nn; ++q) {
+        slow[q] = 1 / v[q];
+        told[q] = 65535;
+        T[q] = 65535;
+    }
+    const int64_t src = (int64_t)(sz - 1) + (int64_t)(sx - 1) * nz;
+    told[src] = 0;
+    for (int it = 0; it < 4 * iMax; ++it) {
+        #pragma omp parallel for schedule(static)
+        for (int64_t j = 0; j < nx; ++j)
+            for (int64_t i = 0; i < nz; ++i) {
+                const double *c = told + j * nz;
+                double a, b, TT;
+                if (i == 0) a = c[1];
+                else if (i == nz - 1) a = c[nz - 2];
+                else a = fmin(c[i - 1], c[i + 1]);
+                if (j == 0) b = told[i + nz];
+                else if (j == nx - 1) b = told[i + (nx - 2) * nz];
+                else b = fmin(told[i + (j - 1) * nz], told[i + (j + 1) * nz]);
+                const double s = slow[i + j * nz];
+                if (fabs(a - b) < s * dx)
+                    TT = 0.5 * (a + b + sqrt((2 * s * s * dx * dx) - (a - b) * (a - b)));
+                else
+                    TT = fmin(a, b) + (s * dx);
+                T[i + j * nz] = fmin(c[i], TT);
+            }
+        T[src] = 0;
+        memcpy(told, T, nn * sizeof(double));
+    }
+    free(slow);
+    free(told);
+    return 0;
+}
+
+/* Kirchhoff migration of one shot: src/migrate.m:1-26 with
+ * src/shot2RecTime.m:1-6 (its nx == size(travelTime,2) branch).
+ *   m = 0;  for each receiver xr in xRecGrid (in order):
+ *       r  = position of xr in xShotAndRecGrid          (ismember, 1-based)
+ *       it = round(T(:,:,ixs)/dt)+1 + round(T(:,:,r)/dt)+1   (1-based sample)
+ *       m  = m + shot(it, xr)
+ * travelTime (nz, nx, ntab) and shot (nt, nx) column-major; ixs, xRecGrid
+ * and xShotAndRecGrid 1-based as in MATLAB.  round() is MATLAB's (half away
+ * from zero), which C's round() is.  A sample index outside 1..nt is an
+ * error, as MATLAB's indexing is. */
+int orc_kirchhoff_migrate(const double *tt, int64_t nz, int64_t nx, int64_t ntab,
+                          const int *xrec, int nrec, const int *xsr, int nsr,
+                          const double *shot, int64_t nt, double dt, int ixs, double *m)
+{
+    if (!tt || !xrec || !xsr || !shot || !m || nz < 1 || nx < 1 || nt < 1 || nrec < 0 ||
+        nsr < 1 || ixs < 1 || ixs > ntab || !(dt > 0))
+        return ORC_EINVAL;
+    const int64_t nn = nz * nx;
+    int *ridx = malloc((nrec > 0 ? nrec : 1) * sizeof(int));
+    if (!ridx) return ORC_ENOMEM;
+    for (int r = 0; r < nrec; ++r) {
+        int k = 0;
+        while (k < nsr && xsr[k] != xrec[r]) ++k;
+        if (k == nsr || k + 1 > ntab || xrec[r] < 1 || xrec[r] > nx) { free(ridx); return ORC_EINVAL; }
+        ridx[r] = k;  /* 0-based table of receiver r */
+    }
+    int bad = 0;
+    const double *Ts = tt + (int64_t)(ixs - 1) * nn;
+    #pragma omp parallel for schedule(static) reduction(|:bad)
+    for (int64_t q = 0; q < nn; ++q) {
+        double acc = 0;
+        for (int r = 0; r < nrec; ++r) {
+            const double *Tr = tt + (int64_t)ridx[r] * nn;
+            const double itd = (round(Ts[q] / dt) + 1) + (round(Tr[q] / dt) + 1);
+            if (!(itd >= 1 && itd <= (double)nt)) { bad = 1; continue; }
+            acc = acc + shot[(int64_t)(xrec[r] - 1) * nt + (int64_t)itd - 1];
+        }
+        m[q] = acc;
+    }
+    free(ridx);
+    return bad ? ORC_EINVAL : 0;
+}

@@ -89,6 +89,18 @@ int orc_rtm_shot_cpml_2d(const double *V, const double *VS, int64_t nz, int64_t 
 int orc_xcorr_image(const double *fwd_snap, const double *rvs_snap, int64_t ncell, int64_t nt,
                     int64_t stride, double eps, double *num, double *den, double *image);
 
+/* Traveltimes by the reference's fast-sweeping eikonal solver
+ * (src/fd_cmex/eikonal2d_mex.cpp:27-267): T (nz*nx, column-major) for a source
+ * at the 1-based cell (sz, sx) after iMax iterations of four sweeps. */
+int orc_eikonal2d(const double *v, int64_t nz, int64_t nx, double dx, int sz, int sx,
+                  int iMax, double *T);
+
+/* Kirchhoff migration of one shot (src/migrate.m:1-26, src/shot2RecTime.m):
+ * m (nz*nx) from travelTime (nz, nx, ntab), 1-based grids and shot (nt, nx). */
+int orc_kirchhoff_migrate(const double *tt, int64_t nz, int64_t nx, int64_t ntab,
+                          const int *xrec, int nrec, const int *xsr, int nsr,
+                          const double *shot, int64_t nt, double dt, int ixs, double *m);
+
 #ifdef __cplusplus
 }
 #endif
