@@ -268,6 +268,51 @@ int sssi_rtm2d_cache_stats(sssi_rtm *r, int *loaded_data, int *loaded_images, in
 int sssi_rtm2d_device_bytes(sssi_rtm *r, double *bytes);
 
 /* ---------------------------------------------------------------------
+ * Kirchhoff path: traveltimes by the fast-sweeping eikonal solver and
+ * Kirchhoff migration (applications/kirchhoff_2d_acoustic/
+ * mainKirchhoffCpmlFor2dAw.m:244-266, :278-293).  Arrays are column-major
+ * FP64; grid positions and source cells are 1-based, as in MATLAB.
+ * ------------------------------------------------------------------- */
+
+/* Replaces eikonal2d_mex (src/fd_cmex/eikonal2d_mex.cpp:27; MATLAB form
+ * src/eikonal2d.m): T = eikonal2d(V, dx, sz, sx, iMax), T (nz, nx).  Same
+ * arithmetic bit for bit, including the reference's integer truncation of dx
+ * (eikonal2d_mex.cpp:35) and its 65535 start value; nz, nx >= 2. */
+int sssi_eikonal2d(const double *V, int64_t nz, int64_t nx, double dx, int sz, int sx, int iMax,
+                   double *T);
+/* The same for nsrc sources at once: T (nz, nx, nsrc). */
+int sssi_eikonal2d_batch(const double *V, int64_t nz, int64_t nx, double dx, int nsrc,
+                         const int *sz, const int *sx, int iMax, double *T);
+
+/* Replaces migrate (src/migrate.m:1 with src/shot2RecTime.m:1):
+ * m = migrate(travelTime, xRecGrid, xShotAndRecGrid, shot, dt, nz, ixs).
+ * travelTime (nz, nx, ntab), shot (nt, nx), m (nz, nx); ixs indexes the
+ * traveltime tables (as the reference's third dimension).  A sample index
+ * outside 1..nt is SSSI_EINVAL (MATLAB's indexing error). */
+int sssi_kirchhoff_migrate(const double *travelTime, int64_t nz, int64_t nx, int64_t ntab,
+                           const int *xRecGrid, int nrec, const int *xShotAndRecGrid, int nsr,
+                           const double *shot, int64_t nt, double dt, int ixs, double *m);
+
+/* Resident form of the app's loops: traveltime tables of every surface
+ * position in xShotAndRecGrid (sources at row 1, as the app's [1 xs]) by the
+ * eikonal solver, kept on the device with their sample indices for dt; then
+ * the migration of many shots, stacked in shot order as the app's
+ * Stacked = Stacked + M (mainKirchhoffCpmlFor2dAw.m:289-291).  No
+ * counterpart call in the reference (its app uses ray2d.m for the tables). */
+typedef struct sssi_kirchhoff sssi_kirchhoff;
+int sssi_kirchhoff_create(const double *V, int64_t nz, int64_t nx, double dx,
+                          const int *xShotAndRecGrid, int nsr, int iMax, double dt,
+                          sssi_kirchhoff **out);
+/* travelTime (nz, nx, nsr). */
+int sssi_kirchhoff_travel_time(sssi_kirchhoff *h, double *travelTime);
+/* shots (nt, nx, nshot), ixs[nshot] table indices (1-based); stack (nz, nx),
+ * images (nz, nx, nshot) or NULL. */
+int sssi_kirchhoff_migrate_shots(sssi_kirchhoff *h, const int *xRecGrid, int nrec,
+                                 const double *shots, int64_t nt, int nshot, const int *ixs,
+                                 double *stack, double *images);
+void sssi_kirchhoff_destroy(sssi_kirchhoff *h);
+
+/* ---------------------------------------------------------------------
  * MAT-file (Level 5) I/O of real double arrays: the durable format of the
  * results above, for callers without MATLAB.  sssi_mat_write writes nvar
  * variables (names[v], data[v] column-major, ndims[v] dimensions each, dims

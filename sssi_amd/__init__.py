@@ -27,10 +27,12 @@ from . import synth
 from .rtm import Rtm2D
 from .serve import Server
 from .matfile import mat_read, mat_write
+from .kirchhoff import eikonal2d, eikonal2d_batch, migrate, Kirchhoff2D
 
 __all__ = [
     "SssiError", "lib", "device_count", "fwdTimeCpmlFor2dAw", "rvsTimeCpmlFor2dAw",
     "fwdTimeCpmlFor3dAw", "fwdTimeSpongeFor2dAw", "rvsTimeSpongeFor2dAw",
     "fwd_cpml_2d_points", "rvs_cpml_2d_ex", "fwd_cpml_3d_points", "fwd_sponge_2d_points",
     "rvs_sponge_2d_ex", "Propagator", "synth", "Rtm2D", "Server", "mat_read", "mat_write",
+    "eikonal2d", "eikonal2d_batch", "migrate", "Kirchhoff2D",
 ]

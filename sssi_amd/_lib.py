@@ -68,6 +68,14 @@ SIGNATURES = {
     "sssi_rtm2d_set_cache": [_vp, ctypes.c_char_p],
     "sssi_rtm2d_cache_stats": [_vp, _I, _I, _I],
     "sssi_rtm2d_device_bytes": [_vp, _D],
+    "sssi_eikonal2d": [_D, _i64, _i64, _dbl, _int, _int, _int, _D],
+    "sssi_eikonal2d_batch": [_D, _i64, _i64, _dbl, _int, _I, _I, _int, _D],
+    "sssi_kirchhoff_migrate": [_D, _i64, _i64, _i64, _I, _int, _I, _int, _D, _i64, _dbl, _int,
+                               _D],
+    "sssi_kirchhoff_create": [_D, _i64, _i64, _dbl, _I, _int, _int, _dbl, ctypes.POINTER(_vp)],
+    "sssi_kirchhoff_travel_time": [_vp, _D],
+    "sssi_kirchhoff_migrate_shots": [_vp, _I, _int, _D, _i64, _int, _I, _D, _D],
+    "sssi_kirchhoff_destroy": [_vp],
     "sssi_mat_write": [ctypes.c_char_p, _int, ctypes.POINTER(ctypes.c_char_p),
                        ctypes.POINTER(_D), _I, ctypes.POINTER(_i64)],
     "sssi_mat_read": [ctypes.c_char_p, ctypes.c_char_p, _I, ctypes.POINTER(_i64), _D, _i64],
@@ -89,7 +97,8 @@ SIGNATURES = {
     "sssi_server_destroy": [_vp],
 }
 _RESTYPE = {"sssi_last_error": ctypes.c_char_p, "sssi_prop_destroy": None,
-            "sssi_rtm2d_destroy": None, "sssi_server_destroy": None}
+            "sssi_rtm2d_destroy": None, "sssi_server_destroy": None,
+            "sssi_kirchhoff_destroy": None}
 
 _lib = None
 
