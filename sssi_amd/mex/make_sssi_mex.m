@@ -9,7 +9,7 @@ inc = fullfile(here, '..', '..', 'include');
 libdir = fullfile(here, '..');
 gateways = {'fwdTimeCpmlFor2dAw_mex', 'rvsTimeCpmlFor2dAw_mex', ...
             'fwdTimeCpmlFor3dAw_mex', 'fwdTimeSpongeFor2dAw_mex', ...
-            'rvsTimeSpongeFor2dAw_mex'};
+            'rvsTimeSpongeFor2dAw_mex', 'eikonal2d_mex', 'migrate_mex'};
 for k = 1:numel(gateways)
     mex('-largeArrayDims', ['-I' inc], ['-L' libdir], '-lsssi', ...
         ['LDFLAGS=$LDFLAGS -Wl,-rpath,' libdir], ...

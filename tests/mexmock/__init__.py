@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 BUILD = os.path.join(HERE, "_build")
 GATEWAYS = ["fwdTimeCpmlFor2dAw_mex", "rvsTimeCpmlFor2dAw_mex", "fwdTimeCpmlFor3dAw_mex",
-            "fwdTimeSpongeFor2dAw_mex", "rvsTimeSpongeFor2dAw_mex"]
+            "fwdTimeSpongeFor2dAw_mex", "rvsTimeSpongeFor2dAw_mex", "eikonal2d_mex",
+            "migrate_mex"]
 
 
 class MexError(Exception):

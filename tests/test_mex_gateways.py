@@ -3,6 +3,9 @@
 CPU: every gateway compiles warning-free and rejects bad calls with a MATLAB
 error identifier before touching the device (argument count, class,
 dimensions, integer ranges, the reference's unchecked 2-D source, options).
+The Kirchhoff gateways (eikonal2d_mex, migrate_mex) reject sources outside
+the grid, non-vector or non-integer positions and a shot whose width differs
+from the tables'.
 GPU: the gateway outputs equal the C-ABI drop-ins bit for bit, the struct
 (point) source equals the dense source, snapStride slices equal the full
 snapshot's, nargout = 1 skips the snapshot, and the gateways match the FP64
@@ -39,7 +42,8 @@ def small_2d(nt=120):
                                         ("rvsTimeCpmlFor2dAw_mex", 7),
                                         ("fwdTimeCpmlFor3dAw_mex", 8),
                                         ("fwdTimeSpongeFor2dAw_mex", 6),
-                                        ("rvsTimeSpongeFor2dAw_mex", 6)])
+                                        ("rvsTimeSpongeFor2dAw_mex", 6),
+                                        ("eikonal2d_mex", 5), ("migrate_mex", 7)])
 def test_argument_count(name, nargs):
     with pytest.raises(MexError) as e:
         call(name, *([1.0] * (nargs - 1)))
@@ -186,3 +190,49 @@ def test_fwd3d_gateway_equals_abi_and_oracle():
     d_p, s_p = call("fwdTimeCpmlFor3dAw_mex", v, pts, 2, 5, 10.0, 10.0, 10.0, 1e-3,
                     {"snapStride": 4.0})
     assert np.array_equal(d_p, d_mex) and np.array_equal(s_p, s_mex[..., ::4])
+
+
+def test_kirchhoff_gateway_validation():
+    V = np.full((10, 12), 2000.0)
+    for k, val, ident in [(2, 0, "sssi:range"), (2, 11, "sssi:range"), (3, 13, "sssi:range"),
+                          (4, -1, "sssi:range"), (1, 0.5, "sssi:range")]:
+        args = [V, 10.0, 1, 1, 3]
+        args[k] = val
+        with pytest.raises(MexError) as e:
+            call("eikonal2d_mex", *args, nargout=1)
+        assert e.value.ident == ident
+    with pytest.raises(MexError) as e:
+        call("eikonal2d_mex", np.ones((1, 12)), 10.0, 1, 1, 3, nargout=1)
+    assert e.value.ident == "sssi:size"
+    tt = np.zeros((10, 12, 12))
+    xs = np.arange(1, 13, dtype=float)
+    shot = np.zeros((50, 12))
+    good = [tt, xs, xs, shot, 1e-3, 10, 1]
+    for k, val, ident in [(1, np.ones((2, 2)), "sssi:size"), (2, xs - 0.5, "sssi:range"),
+                          (3, np.zeros((50, 11)), "sssi:size"), (5, 9, "sssi:size"),
+                          (6, 13, "sssi:range"), (6, 0, "sssi:range")]:
+        args = list(good)
+        args[k] = val
+        with pytest.raises(MexError) as e:
+            call("migrate_mex", *args, nargout=1)
+        assert e.value.ident == ident
+
+
+@pytest.mark.gpu
+def test_kirchhoff_gateways_equal_abi_and_oracle():
+    V = synth.smooth_copy(synth.layered_2d(24, 30, 3, 1500, 3000, seed=4), 2)
+    T = call("eikonal2d_mex", V, 10.0, 1.0, 7.0, 20.0, nargout=1)[0]
+    assert np.array_equal(T, sssi_amd.eikonal2d(V, 10.0, 1, 7, 20))
+    assert np.array_equal(T, oracle.eikonal2d(V, 10.0, 1, 7, 20))
+    xsr = np.arange(1, 31)
+    tt = np.stack([oracle.eikonal2d(V, 10.0, 1, int(x), 20) for x in xsr], axis=2)
+    shot = np.random.default_rng(6).standard_normal((700, 30))
+    xr = xsr[::2]
+    m = call("migrate_mex", tt, xr.astype(float), xsr.astype(float), shot, 1e-3, 24, 9,
+             nargout=1)[0]
+    assert np.array_equal(m, sssi_amd.migrate(tt, xr, xsr, shot, 1e-3, 24, 9))
+    assert np.array_equal(m, oracle.kirchhoff_migrate(tt, xr, xsr, shot, 1e-3, 24, 9))
+    with pytest.raises(MexError) as e:
+        call("migrate_mex", tt, xr.astype(float), xsr.astype(float), shot[:40], 1e-3, 24, 9,
+             nargout=1)
+    assert e.value.ident == "sssi:run"
