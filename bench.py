@@ -21,6 +21,9 @@ Sub-objects of the same line, each with its own fixed step counts:
               batched 16-bit checkpoints, served (shots/hour)
   cfg5_serve  the 60-job Poisson serving mix: p50/p99 latency and throughput,
               continuous batching vs one job at a time
+  kirchhoff   Kirchhoff path in the reference app's geometry: eikonal
+              traveltime tables for every surface position (Gcell-updates/s,
+              roofline of the pass kernel) and migrated shots/hour
 `--workload` runs one of them alone (and the sponge / copy calibrations).
 """
 import argparse
@@ -63,7 +66,7 @@ def cpu_baseline(wl, steps=None, cores=None):
     cores = cores or min(16, os.cpu_count() or 1)
     nz, nx = wl["V"].shape
     if steps is None:
-        steps = max(5, int(10.0 * 2e7 / (nz * nx)))  # ~10 s at ~0.02 Gcell/s per core
+        steps = max(5, int(10.0 * 6e7 / (nz * nx)))  # ~10 s at ~0.06 Gcell/s per core
     steps = min(steps, wl["nt"])
     errors = []
 
@@ -356,6 +359,90 @@ def bench_copy(args):
             "steps": args.steps}
 
 
+def bench_kirchhoff(args):
+    import oracle
+    import sssi_amd
+    from sssi_amd import workloads
+    wl = workloads.kirchhoff()
+    V, shots = wl["V"], wl["shots"]
+    nz, nx = V.shape
+    ntab = wl["xsr"].size
+    nshot = shots.shape[2]
+    t0 = time.perf_counter()
+    k = sssi_amd.Kirchhoff2D(V, wl["d"], wl["xsr"], wl["iMax"], wl["dt"])
+    tables_wall = time.perf_counter() - t0
+    eik_ms, passes, launches, _ = k.timing()
+    t0 = time.perf_counter()
+    k.load_shots(shots)  # records resident, as forward-modelled data on the device would be
+    load_s = time.perf_counter() - t0
+    st = k.migrate_loaded(wl["xRec"], wl["ixs"])  # warm-up
+    steps = max(1, args.steps)
+    mig_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = k.migrate_loaded(wl["xRec"], wl["ixs"])
+        mig_ms += k.timing()[3]
+    mig_wall = time.perf_counter() - t0
+    tt = None if args.no_cpu else k.travel_time()
+    k.close()
+    if not np.all(np.isfinite(st)):
+        raise RuntimeError("non-finite stack")
+    cell_updates = nz * nx * ntab * passes
+    # per launch (H passes on overlapping tiles): each table read and written once, slowness
+    launch_bytes = nz * nx * (16 * ntab + 8)
+    avg_launch_s = eik_ms / 1e3 / launches
+    sums = nz * nx * wl["xRec"].size * nshot  # trace samples summed per migrate call
+    out = {
+        "metric": "Kirchhoff migrated shots/hour (tables and records resident in HBM)",
+        "value": round(nshot * steps / mig_wall * 3600, 1),
+        "unit": "shots/h",
+        "kernel_shots_per_hour": round(nshot * steps / (mig_ms / 1e3) * 3600, 1),
+        "records_upload_s": round(load_s, 3),
+        "migrate_gsums_per_s": round(sums * steps / (mig_ms / 1e3) / 1e9, 2),
+        "steps": steps,
+        "dtype": "f64",
+        "config": {"workload": f"Kirchhoff 2D, {nz}x{nx}, d=10m, {ntab} tables (every surface "
+                               f"position), {wl['xRec'].size} receivers, {nshot} shots per "
+                               f"migrate call, nt={wl['nt']}, iMax={wl['iMax']}",
+                   "nz": nz, "nx": nx, "tables": ntab, "receivers": int(wl["xRec"].size),
+                   "shots": nshot, "nt": wl["nt"], "iMax": wl["iMax"]},
+        "eikonal": {"tables_wall_s": round(tables_wall, 3), "kernel_s": round(eik_ms / 1e3, 3),
+                    "passes": passes, "gcell_updates_per_s": round(cell_updates / (eik_ms / 1e3)
+                                                                   / 1e9, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(launch_bytes / avg_launch_s / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(launch_bytes / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "eik_tile<4,32>",
+                     "alg_bytes_per_launch": launch_bytes,
+                     "avg_launch_us": round(avg_launch_s * 1e6, 2), "launches": launches,
+                     "note": "4 passes per launch on 64x32 tiles: the kernel is bound by FP64 "
+                             "VALU issue (the update with a correctly rounded sqrt, plus halo "
+                             "recompute), not by HBM"},
+    }
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        oracle.set_threads(cores)
+        n_tab = 12  # ~5 s: full-length tables of 12 surface positions
+        t0 = time.perf_counter()
+        for x in np.linspace(1, nx, n_tab).round().astype(int):
+            oracle.eikonal2d(V, wl["d"], 1, int(x), wl["iMax"])
+        e_el = time.perf_counter() - t0
+        nmig = 6 * nshot  # ~5 s: the loaded shots six times over
+        t0 = time.perf_counter()
+        for s in range(nmig):
+            oracle.kirchhoff_migrate(tt, wl["xRec"], wl["xsr"], shots[:, :, s % nshot], wl["dt"],
+                                     nz, int(wl["ixs"][s % nshot]))
+        m_el = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": round(nmig / m_el * 3600, 1), "unit": "shots/h", "cores": cores,
+            "kind": "port",
+            "eikonal_gcell_updates_per_s": round(nz * nx * n_tab * passes / e_el / 1e9, 4),
+            "sample": f"FP64 oracle on {cores} OpenMP threads: migration of {nmig} shots "
+                      f"against {wl['xRec'].size} receivers ({m_el:.1f} s); eikonal tables of "
+                      f"{n_tab} surface positions, {passes} passes each ({e_el:.1f} s)"}
+    return out
+
+
 def _sub(fn, args, **over):
     """Run one sub-benchmark with its own fixed step counts; a failure is
     recorded instead of ending the run."""
@@ -450,7 +537,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="2d/3d: default kernel layout instead of the autotuned one")
     ap.add_argument("--workload", default="2d",
-                    choices=["2d", "3d", "sponge", "rtm", "serve", "copy"],
+                    choices=["2d", "3d", "sponge", "rtm", "serve", "copy", "kirchhoff"],
                     help="2d: the headline 2-D forward line with the sub-benchmarks (default); "
                          "the others: one workload alone")
     ap.add_argument("--nt", type=int, default=None,
@@ -466,7 +553,7 @@ def main():
               "not part of this build.", file=sys.stderr)
         sys.exit(2)
     one = {"3d": bench_3d, "sponge": bench_sponge, "rtm": bench_rtm, "serve": bench_serve,
-           "copy": bench_copy}
+           "copy": bench_copy, "kirchhoff": bench_kirchhoff}
     if args.workload in one:
         print(json.dumps(one[args.workload](args)), flush=True)
         return
@@ -477,6 +564,7 @@ def main():
         out["cfg4_3d"] = _sub(bench_3d, args, steps=2, warmup=1, nt=None)
         out["cfg3_rtm"] = _sub(bench_rtm, args, shots=64)
         out["cfg5_serve"] = _sub(bench_serve, args, jobs=60, rate=20.0)
+        out["kirchhoff"] = _sub(bench_kirchhoff, args, steps=3)
     print(json.dumps(out), flush=True)
 
 

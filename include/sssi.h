@@ -310,6 +310,17 @@ int sssi_kirchhoff_travel_time(sssi_kirchhoff *h, double *travelTime);
 int sssi_kirchhoff_migrate_shots(sssi_kirchhoff *h, const int *xRecGrid, int nrec,
                                  const double *shots, int64_t nt, int nshot, const int *ixs,
                                  double *stack, double *images);
+/* The same in two halves, for records that stay on the device: load the
+ * records (nt, nx, nshot) into the handle (replacing any loaded before), then
+ * migrate the loaded records as often as needed (ixs[nshot]). */
+int sssi_kirchhoff_load_shots(sssi_kirchhoff *h, const double *shots, int64_t nt, int nshot);
+int sssi_kirchhoff_migrate_loaded(sssi_kirchhoff *h, const int *xRecGrid, int nrec,
+                                  const int *ixs, double *stack, double *images);
+/* Kernel time (HIP events on the handle's stream) of the eikonal passes at
+ * create, their count and the launches that ran them, and the kernel time of
+ * the last migration. */
+int sssi_kirchhoff_timing(sssi_kirchhoff *h, double *eikonal_ms, int64_t *passes,
+                          int64_t *launches, double *migrate_ms);
 void sssi_kirchhoff_destroy(sssi_kirchhoff *h);
 
 /* ---------------------------------------------------------------------

@@ -75,6 +75,9 @@ SIGNATURES = {
     "sssi_kirchhoff_create": [_D, _i64, _i64, _dbl, _I, _int, _int, _dbl, ctypes.POINTER(_vp)],
     "sssi_kirchhoff_travel_time": [_vp, _D],
     "sssi_kirchhoff_migrate_shots": [_vp, _I, _int, _D, _i64, _int, _I, _D, _D],
+    "sssi_kirchhoff_load_shots": [_vp, _D, _i64, _int],
+    "sssi_kirchhoff_migrate_loaded": [_vp, _I, _int, _I, _D, _D],
+    "sssi_kirchhoff_timing": [_vp, _D, ctypes.POINTER(_i64), ctypes.POINTER(_i64), _D],
     "sssi_kirchhoff_destroy": [_vp],
     "sssi_mat_write": [ctypes.c_char_p, _int, ctypes.POINTER(ctypes.c_char_p),
                        ctypes.POINTER(_D), _I, ctypes.POINTER(_i64)],

@@ -19,6 +19,8 @@
 // reference's `Stacked = Stacked + M`.  An index outside the record is an
 // error, as MATLAB's indexing is.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -33,7 +35,23 @@ static unsigned grid_blocks(int64_t n) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(8192, (n + 255) / 256));
 }
 
-// One Jacobi pass for nsrc tables [k][j][i] (column-major per table).
+// The reference's cell update (eikonal2d_mex.cpp:80-105 and its three
+// repeats): a = min of the z neighbours, b = min of the x neighbours (the
+// single inner neighbour at a grid edge), Godunov update, min with the old
+// value.  Contraction is off at every caller so the operation order, and with
+// it every bit, is the reference's.
+__device__ __forceinline__ double eik_update(double me, double a, double b, double s, double dx) {
+#pragma clang fp contract(off)
+    double TT;
+    if (fabs(a - b) < s * dx)
+        TT = 0.5 * (a + b + sqrt((2 * s * s * dx * dx) - (a - b) * (a - b)));
+    else
+        TT = fmin(a, b) + (s * dx);
+    return fmin(me, TT);
+}
+
+// One Jacobi pass for nsrc tables [k][j][i] (column-major per table): the
+// plain form, one launch per pass.
 __global__ void __launch_bounds__(256) eik_pass(const double *__restrict__ slow,
                                                 const double *__restrict__ told,
                                                 double *__restrict__ tnew, int64_t nz, int64_t nx,
@@ -49,20 +67,120 @@ __global__ void __launch_bounds__(256) eik_pass(const double *__restrict__ slow,
         const int64_t j = q / nz, i = q - j * nz;
         const double *T = told + (int64_t)k * nn;
         const double *c = T + j * nz;
-        double a, b, TT;
+        double a, b;
         if (i == 0) a = c[1];
         else if (i == nz - 1) a = c[nz - 2];
         else a = fmin(c[i - 1], c[i + 1]);
         if (j == 0) b = T[i + nz];
         else if (j == nx - 1) b = T[i + (nx - 2) * nz];
         else b = fmin(T[i + (j - 1) * nz], T[i + (j + 1) * nz]);
-        const double s = slow[q];
-        if (fabs(a - b) < s * dx)
-            TT = 0.5 * (a + b + sqrt((2 * s * s * dx * dx) - (a - b) * (a - b)));
-        else
-            TT = fmin(a, b) + (s * dx);
-        const double t = fmin(c[i], TT);
+        const double t = eik_update(c[i], a, b, slow[q], dx);
         tnew[g] = q == src_cell[k] ? 0.0 : t;
+    }
+}
+
+// 64-bit wave shifts by one lane (DPP wave_shr:1 / wave_shl:1 on both
+// dwords): lane l receives lane l-1's (shr) or lane l+1's (shl) value; the
+// lane with no source lane reads 0 (bound_ctrl).
+__device__ __forceinline__ double wave_shr1(double v) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)u, 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), 0x138, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_shl1(double v) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)u, 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), 0x130, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Temporally blocked passes.  A block holds a 64 x TW tile of one table
+// (rows z0-H .. z0-H+63, columns x0-H .. x0-H+TW-1) and runs up to H Jacobi
+// passes on it, then writes the centre (64-2H) x (TW-2H) cells.  A pass at a
+// tile edge that is not a grid edge reads a clamped neighbour and goes wrong,
+// and each pass moves that error one cell inwards, so after npass <= H passes
+// the centre is exactly the plain form's; a pass skips the columns that can
+// no longer reach the centre (H >= npass keeps every read column in the
+// tile).  Grid edges use the reference's inner-neighbour rule on in-tile
+// cells; cells outside the grid are computed but never read by a cell inside
+// it and never stored.
+//   lane = tile row (z neighbours by DPP wave shifts of the lane's own
+//   register), wave w owns tile columns w, w+4, ... (x neighbours from the
+//   LDS copy of the previous level); two barriers per pass.
+// The launch is 1-D over ntiles * nsrc blocks, XCD-banded: block b runs on XCD
+// b % 8 and takes the b/8-th tile of that XCD's contiguous band, so
+// neighbouring tiles of a table share one L2 for their halos.
+template <int H, int TW>
+__global__ void __launch_bounds__(256) eik_tile(const double *__restrict__ slow,
+                                                const double *__restrict__ told,
+                                                double *__restrict__ tnew, int nz, int nx,
+                                                int ntz, int ntiles,
+                                                const int64_t *__restrict__ src_cell, double dx,
+                                                int npass) {
+#pragma clang fp contract(off)
+    constexpr int C = TW / 4;  // columns per wave
+    constexpr int OZ = 64 - 2 * H, OX = TW - 2 * H;
+    __shared__ double L[TW * 64];
+    const int lz = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int qb = nb / 8, rem = nb % 8, xc = b % 8;
+    const int t = xc * qb + min(xc, rem) + b / 8;
+    const int k = t / ntiles, tile = t - k * ntiles;
+    const int tx = tile / ntz, tz = tile - tx * ntz;
+    const int gi = tz * OZ - H + lz;
+    const int x0 = tx * OX - H;
+    const int nn = nz * nx;
+    const double *T = told + (int64_t)k * nn;
+    double *TN = tnew + (int64_t)k * nn;
+    const int src = (int)src_cell[k];
+    const bool zin = gi >= 0 && gi < nz;
+
+    double tv[C], sv[C];
+    unsigned srcmask = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int lx = w + 4 * c, gj = x0 + lx;
+        const bool in = zin && gj >= 0 && gj < nx;
+        const int q = gj * nz + gi;
+        tv[c] = in ? T[q] : kT0;
+        sv[c] = in ? slow[q] : 0.0;
+        if (in && q == src) srcmask |= 1u << c;
+        L[lx * 64 + lz] = tv[c];
+    }
+    __syncthreads();
+    for (int p = 0; p < npass; ++p) {
+        // columns whose value can still reach the centre by the last pass
+        const int r = npass - 1 - p;
+        double tn[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int lx = w + 4 * c, gj = x0 + lx;
+            tn[c] = tv[c];
+            if (lx < H - r || lx >= H + OX + r) continue;  // wave-uniform
+            const double me = tv[c];
+            // edge rule as neighbour substitution: min(x, x) = x
+            const double up0 = wave_shr1(me), dn0 = wave_shl1(me);
+            const double up = gi == 0 ? dn0 : up0, dn = gi == nz - 1 ? up0 : dn0;
+            const int cl = gj == 0 ? lx + 1 : lx - 1, cr = gj == nx - 1 ? lx - 1 : lx + 1;
+            const double a = fmin(up, dn);
+            const double bb = fmin(L[cl * 64 + lz], L[cr * 64 + lz]);
+            const double v = eik_update(me, a, bb, sv[c], dx);
+            tn[c] = (srcmask >> c) & 1u ? 0.0 : v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            tv[c] = tn[c];
+            L[(w + 4 * c) * 64 + lz] = tn[c];
+        }
+        __syncthreads();
+    }
+    if (lz < H || lz >= H + OZ || !zin) return;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int lx = w + 4 * c, gj = x0 + lx;
+        if (lx >= H && lx < H + OX && gj < nx) TN[gj * nz + gi] = tv[c];
     }
 }
 
@@ -89,7 +207,11 @@ __global__ void eik_index(const double *__restrict__ t, int64_t n, double dt,
 }
 
 // Stack (and per-shot images) over nshot shots: pixel q, shot s (table ts[s]),
-// receivers r in order (table tr[r], record column xr[r]).
+// receivers r in order (table tr[r], record column xr[r]).  A thread owns a
+// pixel and takes the shots S at a time, so each receiver's index is loaded
+// once per S shots; every shot keeps its own sum over the receivers in order
+// and the stack adds the shots in order -- the reference's sums exactly.
+template <int S>
 __global__ void __launch_bounds__(256) kirch_stack(const int *__restrict__ kidx, int64_t nn,
                                                    const int *__restrict__ ts, int nshot,
                                                    const int *__restrict__ tr,
@@ -102,24 +224,83 @@ __global__ void __launch_bounds__(256) kirch_stack(const int *__restrict__ kidx,
          q += (int64_t)gridDim.x * blockDim.x) {
         double st = 0.0;
         bool oob = false;
-        for (int s = 0; s < nshot; ++s) {
-            const int ks = kidx[(int64_t)ts[s] * nn + q];
-            const double *rec = shots + (int64_t)s * nt * ncol;
-            double m = 0.0;
-            for (int r = 0; r < nrec; ++r) {
-                const int64_t it = (int64_t)ks + kidx[(int64_t)tr[r] * nn + q];  // 1-based
-                if (it < 1 || it > nt) {
-                    oob = true;
-                    continue;
-                }
-                m = m + rec[(int64_t)xr[r] * nt + (it - 1)];
+        for (int s0 = 0; s0 < nshot; s0 += S) {
+            const int ns = min(S, nshot - s0);
+            int ks[S];
+            double m[S];
+#pragma unroll
+            for (int u = 0; u < S; ++u) {
+                ks[u] = u < ns ? kidx[(int64_t)ts[s0 + u] * nn + q] : 0;
+                m[u] = 0.0;
             }
-            if (images) images[(int64_t)s * nn + q] = m;
-            st = st + m;
+            for (int r = 0; r < nrec; ++r) {
+                const int64_t kr = kidx[(int64_t)tr[r] * nn + q];
+                const double *col = shots + (int64_t)s0 * nt * ncol + (int64_t)xr[r] * nt - 1;
+#pragma unroll
+                for (int u = 0; u < S; ++u) {
+                    if (u >= ns) break;
+                    const int64_t it = ks[u] + kr;  // 1-based sample
+                    if (it < 1 || it > nt) {
+                        oob = true;
+                        continue;
+                    }
+                    m[u] = m[u] + col[(int64_t)u * nt * ncol + it];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < S; ++u) {
+                if (u >= ns) break;
+                if (images) images[(int64_t)(s0 + u) * nn + q] = m[u];
+                st = st + m[u];
+            }
         }
         stack[q] = st;
         if (oob) atomicOr(bad, 1);
     }
+}
+constexpr int kShotBatch = 4;
+
+// Pass kernel choice: H passes per launch on 64 x TW tiles, or (h = 0) the
+// plain one-pass kernel.  SSSI_EIK = "plain" | "<H>x<TW>" (4x32, 4x64, 8x48,
+// 8x64) overrides the default 4x32.  Cell-passes per second on 600 tables of
+// a 300 x 600 grid, 1800 passes: plain 148 G, 4x32 369 G, 8x48 315 G,
+// 4x64 273 G, 8x64 261 G (the wider tiles hold 16 columns per wave in
+// registers, 176+ VGPRs, and run 2 waves per SIMD).
+struct EikLayout {
+    int h = 4, tw = 32;
+    void launch(dim3 grid, hipStream_t s, const double *slow, const double *told, double *tnew,
+                int nz, int nx, int ntz, int ntiles, const int64_t *src, double dx,
+                int np) const {
+#define SSSI_EIK_CASE(H_, TW_)                                                               \
+    if (h == H_ && tw == TW_) {                                                              \
+        hipLaunchKernelGGL((eik_tile<H_, TW_>), grid, dim3(256), 0, s, slow, told, tnew, nz, nx, \
+                           ntz, ntiles, src, dx, np);                                        \
+        return;                                                                              \
+    }
+        SSSI_EIK_CASE(4, 32)
+        SSSI_EIK_CASE(4, 64)
+        SSSI_EIK_CASE(8, 48)
+        SSSI_EIK_CASE(8, 64)
+#undef SSSI_EIK_CASE
+        fail(SSSI_EINVAL, "eikonal layout %dx%d not built", h, tw);
+    }
+};
+
+static EikLayout eik_layout() {
+    EikLayout l;
+    const char *e = getenv("SSSI_EIK");
+    if (!e || !*e) return l;
+    if (!strcmp(e, "plain")) {
+        l.h = 0;
+        return l;
+    }
+    int h = 0, tw = 0;
+    if (sscanf(e, "%dx%d", &h, &tw) == 2) {
+        l.h = h;
+        l.tw = tw;
+        if ((h == 4 && (tw == 32 || tw == 64)) || (h == 8 && (tw == 48 || tw == 64))) return l;
+    }
+    fail(SSSI_EINVAL, "SSSI_EIK must be plain, 4x32, 4x64, 8x48 or 8x64 (got \"%s\")", e);
 }
 
 // ---------------------------------------------------------------------------
@@ -132,6 +313,8 @@ class Eikonal2D {
     DevBuf<double> slow, t[2];
     DevBuf<int64_t> src;
     int cur = 0;
+    int64_t passes = 0, launches = 0;
+    float pass_ms = 0;  // kernel time of the passes (HIP events on `stream`)
 
     Eikonal2D(const double *v, int64_t nz_, int64_t nx_, double dx, int nsrc_, const int *sz,
               const int *sx, int iMax)
@@ -163,13 +346,40 @@ class Eikonal2D {
         SSSI_HIP(hipGetLastError());
         cur = 0;
         const double dxi = (double)(int)dx;  // eikonal2d_mex.cpp:35 reads dx as an int
-        for (int p = 0; p < 4 * iMax; ++p) {
-            hipLaunchKernelGGL(eik_pass, dim3(grid_blocks(total)), dim3(256), 0, stream, slow.p,
-                               t[cur].p, t[cur ^ 1].p, nz, nx, nsrc, src.p, dxi);
-            SSSI_HIP(hipGetLastError());
-            cur ^= 1;
+        hipEvent_t ev[2];
+        SSSI_HIP(hipEventCreate(&ev[0]));
+        SSSI_HIP(hipEventCreate(&ev[1]));
+        SSSI_HIP(hipEventRecord(ev[0], stream));
+        const EikLayout lay = eik_layout();
+        if (lay.h == 0) {
+            for (int p = 0; p < 4 * iMax; ++p) {
+                hipLaunchKernelGGL(eik_pass, dim3(grid_blocks(total)), dim3(256), 0, stream,
+                                   slow.p, t[cur].p, t[cur ^ 1].p, nz, nx, nsrc, src.p, dxi);
+                SSSI_HIP(hipGetLastError());
+                cur ^= 1;
+            }
+            launches = 4 * (int64_t)iMax;
+        } else {
+            const int oz = 64 - 2 * lay.h, ox = lay.tw - 2 * lay.h;
+            const int ntz = (int)((nz + oz - 1) / oz), ntx = (int)((nx + ox - 1) / ox);
+            const int64_t nblk = (int64_t)ntz * ntx * nsrc;
+            if (nblk > INT32_MAX) fail(SSSI_EINVAL, "too many traveltime tables for one launch");
+            launches = 0;
+            for (int p = 0; p < 4 * iMax; p += lay.h) {
+                const int np = std::min(lay.h, 4 * iMax - p);
+                lay.launch(dim3((unsigned)nblk), stream, slow.p, t[cur].p, t[cur ^ 1].p, (int)nz,
+                           (int)nx, ntz, ntz * ntx, src.p, dxi, np);
+                SSSI_HIP(hipGetLastError());
+                cur ^= 1;
+                ++launches;
+            }
         }
+        passes = 4 * (int64_t)iMax;
+        SSSI_HIP(hipEventRecord(ev[1], stream));
         SSSI_HIP(hipStreamSynchronize(stream));
+        SSSI_HIP(hipEventElapsedTime(&pass_ms, ev[0], ev[1]));
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
     }
     ~Eikonal2D() {
         if (stream) (void)hipStreamDestroy(stream);
@@ -200,12 +410,13 @@ static std::vector<int> receiver_tables(const int *xrec, int nrec, const int *xs
     return tr;
 }
 
-// Migration of nshot records against resident index tables.
-static void kirchhoff_run(const int *kidx_dev, int64_t nz, int64_t nx, int64_t ntab,
-                          const int *xrec, int nrec, const int *xsr, int nsr,
-                          const double *shots, int64_t nt, int nshot, const int *ixs,
-                          double *stack, double *images, hipStream_t s) {
-    if (nt < 1 || nshot < 1 || !shots || !ixs || !stack || nrec < 0 || (nrec > 0 && !xrec))
+// Migration of nshot device-resident records against resident index tables;
+// returns the kernel time (HIP events on stream s).
+static float kirchhoff_run(const int *kidx_dev, int64_t nz, int64_t nx, int64_t ntab,
+                           const int *xrec, int nrec, const int *xsr, int nsr,
+                           const double *dshots, int64_t nt, int nshot, const int *ixs,
+                           double *stack, double *images, hipStream_t s) {
+    if (nt < 1 || nshot < 1 || !dshots || !ixs || !stack || nrec < 0 || (nrec > 0 && !xrec))
         fail(SSSI_EINVAL, "kirchhoff: bad arguments");
     std::vector<int> tr = receiver_tables(xrec, nrec, xsr, nsr, ntab, nx);
     std::vector<int> ts(nshot), xr0(std::max(nrec, 1));
@@ -218,27 +429,36 @@ static void kirchhoff_run(const int *kidx_dev, int64_t nz, int64_t nx, int64_t n
     for (int r = 0; r < nrec; ++r) xr0[r] = xrec[r] - 1;
     const int64_t nn = nz * nx;
     DevBuf<int> dts(nshot), dtr(std::max(nrec, 1)), dxr(std::max(nrec, 1)), bad(1);
-    DevBuf<double> dshots((size_t)nt * nx * nshot), dstack(nn), dimg(images ? (size_t)nn * nshot : 0);
+    DevBuf<double> dstack(nn), dimg(images ? (size_t)nn * nshot : 0);
     SSSI_HIP(hipMemcpyAsync(dts.p, ts.data(), dts.bytes(), hipMemcpyHostToDevice, s));
     if (nrec > 0) {
         SSSI_HIP(hipMemcpyAsync(dtr.p, tr.data(), (size_t)nrec * 4, hipMemcpyHostToDevice, s));
         SSSI_HIP(hipMemcpyAsync(dxr.p, xr0.data(), (size_t)nrec * 4, hipMemcpyHostToDevice, s));
     }
-    SSSI_HIP(hipMemcpyAsync(dshots.p, shots, dshots.bytes(), hipMemcpyHostToDevice, s));
     SSSI_HIP(hipMemsetAsync(bad.p, 0, 4, s));
-    hipLaunchKernelGGL(kirch_stack, dim3(grid_blocks(nn)), dim3(256), 0, s, kidx_dev, nn, dts.p,
-                       nshot, dtr.p, dxr.p, nrec, dshots.p, nt, nx, dstack.p,
+    hipEvent_t ev[2];
+    SSSI_HIP(hipEventCreate(&ev[0]));
+    SSSI_HIP(hipEventCreate(&ev[1]));
+    SSSI_HIP(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(kirch_stack<kShotBatch>, dim3(grid_blocks(nn)), dim3(256), 0, s, kidx_dev,
+                       nn, dts.p, nshot, dtr.p, dxr.p, nrec, dshots, nt, nx, dstack.p,
                        images ? dimg.p : nullptr, bad.p);
     SSSI_HIP(hipGetLastError());
+    SSSI_HIP(hipEventRecord(ev[1], s));
     int hbad = 0;
     SSSI_HIP(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
     SSSI_HIP(hipMemcpyAsync(stack, dstack.p, dstack.bytes(), hipMemcpyDeviceToHost, s));
     if (images)
         SSSI_HIP(hipMemcpyAsync(images, dimg.p, dimg.bytes(), hipMemcpyDeviceToHost, s));
     SSSI_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    SSSI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
     if (hbad)
         fail(SSSI_EINVAL, "a traveltime sample index falls outside the %lld-sample record "
              "(the reference's migrate indexes past the shot there)", (long long)nt);
+    return ms;
 }
 
 // Traveltimes of surface positions and their sample indices, resident.
@@ -249,6 +469,29 @@ class Kirchhoff2D {
     double dt = 0;
     std::unique_ptr<Eikonal2D> eik;
     DevBuf<int> kidx;
+    DevBuf<double> rec;  // loaded records (nt, nx, nrec_shots), column-major
+    int64_t rec_nt = 0;
+    int rec_shots = 0;
+    float migrate_ms = 0;  // kernel time of the last migration
+
+    void load(const double *shots, int64_t nt, int nshot) {
+        if (!shots || nt < 1 || nshot < 1) fail(SSSI_EINVAL, "kirchhoff: no records to load");
+        const size_t n = (size_t)nt * (size_t)nx * (size_t)nshot;
+        if (rec.n != n) {
+            rec_shots = 0;
+            rec.alloc(n);  // frees the previous records first
+        }
+        SSSI_HIP(hipMemcpyAsync(rec.p, shots, rec.bytes(), hipMemcpyHostToDevice, eik->stream));
+        SSSI_HIP(hipStreamSynchronize(eik->stream));
+        rec_nt = nt;
+        rec_shots = nshot;
+    }
+    void migrate(const int *xrec, int nrec, const int *ixs, double *stack, double *images) {
+        if (rec_shots < 1) fail(SSSI_ESTATE, "kirchhoff: no records loaded");
+        migrate_ms = kirchhoff_run(kidx.p, nz, nx, (int64_t)xsr.size(), xrec, nrec, xsr.data(),
+                                   (int)xsr.size(), rec.p, rec_nt, rec_shots, ixs, stack,
+                                   images, eik->stream);
+    }
 
     Kirchhoff2D(const double *v, int64_t nz_, int64_t nx_, double dx, const int *xsr_, int nsr,
                 int iMax, double dt_)
@@ -339,8 +582,11 @@ int sssi_kirchhoff_migrate(const double *travelTime, int64_t nz, int64_t nx, int
             hipLaunchKernelGGL(eik_index, dim3(grid_blocks(n)), dim3(256), 0, s, tt.p, n, dt,
                                kidx.p);
             SSSI_HIP(hipGetLastError());
-            kirchhoff_run(kidx.p, nz, nx, ntab, xRecGrid, nrec, xShotAndRecGrid, nsr, shot, nt, 1,
-                          &ixs, m, nullptr, s);
+            if (!shot || nt < 1) fail(SSSI_EINVAL, "shot must not be empty");
+        DevBuf<double> rec((size_t)nt * nx);
+        SSSI_HIP(hipMemcpyAsync(rec.p, shot, rec.bytes(), hipMemcpyHostToDevice, s));
+        kirchhoff_run(kidx.p, nz, nx, ntab, xRecGrid, nrec, xShotAndRecGrid, nsr, rec.p, nt, 1,
+                      &ixs, m, nullptr, s);
         } catch (...) {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
@@ -374,9 +620,34 @@ int sssi_kirchhoff_migrate_shots(sssi_kirchhoff *h, const int *xRecGrid, int nre
                                  double *stack, double *images) {
     return kguard([&] {
         if (!h || !h->impl) fail(SSSI_EINVAL, "NULL handle");
-        Kirchhoff2D &k = *h->impl;
-        kirchhoff_run(k.kidx.p, k.nz, k.nx, (int64_t)k.xsr.size(), xRecGrid, nrec, k.xsr.data(),
-                      (int)k.xsr.size(), shots, nt, nshot, ixs, stack, images, k.eik->stream);
+        h->impl->load(shots, nt, nshot);
+        h->impl->migrate(xRecGrid, nrec, ixs, stack, images);
+    });
+}
+
+int sssi_kirchhoff_load_shots(sssi_kirchhoff *h, const double *shots, int64_t nt, int nshot) {
+    return kguard([&] {
+        if (!h || !h->impl) fail(SSSI_EINVAL, "NULL handle");
+        h->impl->load(shots, nt, nshot);
+    });
+}
+
+int sssi_kirchhoff_migrate_loaded(sssi_kirchhoff *h, const int *xRecGrid, int nrec,
+                                  const int *ixs, double *stack, double *images) {
+    return kguard([&] {
+        if (!h || !h->impl) fail(SSSI_EINVAL, "NULL handle");
+        h->impl->migrate(xRecGrid, nrec, ixs, stack, images);
+    });
+}
+
+int sssi_kirchhoff_timing(sssi_kirchhoff *h, double *eikonal_ms, int64_t *passes,
+                          int64_t *launches, double *migrate_ms) {
+    return kguard([&] {
+        if (!h || !h->impl) fail(SSSI_EINVAL, "NULL handle");
+        if (eikonal_ms) *eikonal_ms = h->impl->eik->pass_ms;
+        if (passes) *passes = h->impl->eik->passes;
+        if (launches) *launches = h->impl->eik->launches;
+        if (migrate_ms) *migrate_ms = h->impl->migrate_ms;
     });
 }
 

@@ -78,6 +78,35 @@ class Kirchhoff2D:
         _call("sssi_kirchhoff_travel_time", self._h, _dp(T))
         return T
 
+    def timing(self):
+        """(eikonal kernel ms, passes, launches, last migration kernel ms), from
+        HIP events on the handle's stream."""
+        e, m = ctypes.c_double(), ctypes.c_double()
+        n, nl = ctypes.c_int64(), ctypes.c_int64()
+        _call("sssi_kirchhoff_timing", self._h, ctypes.byref(e), ctypes.byref(n),
+              ctypes.byref(nl), ctypes.byref(m))
+        return e.value, n.value, nl.value, m.value
+
+    def load_shots(self, shots):
+        """Keep records (nt, nx, nshot) on the device for migrate_loaded."""
+        s = _f64(shots)
+        if s.ndim == 2:
+            s = s[:, :, None]
+        _call("sssi_kirchhoff_load_shots", self._h, _dp(s), s.shape[0], s.shape[2])
+        self._loaded = s.shape[2]
+
+    def migrate_loaded(self, xRecGrid, ixs, images=False):
+        """Migrate the loaded records: stack (nz, nx) [, images (nz, nx, nshot)]."""
+        ix = _ints(ixs)
+        if ix.size != getattr(self, "_loaded", -1):
+            raise ValueError("one table index per loaded shot")
+        xr = _ints(xRecGrid)
+        st = np.zeros((self.nz, self.nx), order="F")
+        im = np.zeros((self.nz, self.nx, ix.size), order="F") if images else None
+        _call("sssi_kirchhoff_migrate_loaded", self._h, _ip(xr), xr.size, _ip(ix), _dp(st),
+              _dp(im))
+        return (st, im) if images else st
+
     def migrate_shots(self, xRecGrid, shots, ixs, images=False):
         """shots (nt, nx, nshot) with table indices ixs (1-based): the stack
         (nz, nx) and, if asked, the per-shot images (nz, nx, nshot)."""
@@ -92,4 +121,5 @@ class Kirchhoff2D:
         im = np.zeros((self.nz, self.nx, ix.size), order="F") if images else None
         _call("sssi_kirchhoff_migrate_shots", self._h, _ip(xr), xr.size, _dp(s), s.shape[0],
               ix.size, _ip(ix), _dp(st), _dp(im))
+        self._loaded = ix.size
         return (st, im) if images else st

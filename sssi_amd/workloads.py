@@ -16,6 +16,12 @@ tests and the benchmark.
   config5(n)     serving mix: 70 % 2-D forward / RTM-shot jobs (200..2000 x
                  200..4000, N 2..5, nt 500..3000), 30 % 3-D forward (128^3 ..
                  256^3, nt 500..1500); layered 1500-4500 m/s; 10-30 Hz Ricker
+  kirchhoff(n)   Kirchhoff migration in the geometry of the reference's
+                 Kirchhoff app (shots and receivers at every surface column,
+                 tables for every position): 300 x 600, d 10 m, velocity
+                 increasing with depth 1800-4000 m/s (10-cell blur), the app's
+                 dt and nt, iMax (nz + nx) / 2; n shots (16) spread over the
+                 surface with seeded synthetic records
 """
 import numpy as np
 
@@ -118,3 +124,23 @@ def job_cells(kind, kw):
         return int(np.prod(kw["v"].shape)) * kw["nt"]
     v = kw["v"] if kind == "fwd2d" else kw["V"]
     return int(np.prod(v.shape)) * kw["nt"] * (3 if kind == "rtm2d" else 1)
+
+
+def kirchhoff(nshots=16, seed=31):
+    """The app's rules (applications/kirchhoff_2d_acoustic/
+    mainKirchhoffCpmlFor2dAw.m:51-85, :111, :115): xShotGrid = xRecGrid =
+    1..nx, xShotAndRecGrid their union, dt = 0.75 dz / vmax / sqrt(2),
+    nt = round(sqrt((dx nx)^2 + (dz nz)^2) * 2 / vmin / dt + 1)."""
+    nz, nx, d = 300, 600, 10.0
+    vmin, vmax = 1800.0, 4000.0
+    v = np.sort(synth.layered_2d(nz, nx, 10, vmin, vmax, seed=seed), axis=0)
+    V = np.clip(synth.smooth_copy(v, 10), vmin, vmax)
+    dt = 0.75 * (d / vmax / np.sqrt(2))
+    nt = int(round(np.sqrt((d * nx) ** 2 + (d * nz) ** 2) * 2 / vmin / dt + 1))
+    xsr = np.arange(1, nx + 1)
+    xShot = np.unique(np.round(np.linspace(1, nx, nshots)).astype(int))
+    rng = np.random.default_rng(seed + 1)
+    shots = np.asfortranarray(
+        rng.standard_normal((nt, nx, xShot.size), dtype=np.float32).astype(np.float64))
+    return dict(V=V, d=d, dt=dt, nt=nt, iMax=(nz + nx) // 2, xsr=xsr, xRec=xsr, xShot=xShot,
+                ixs=xShot, shots=shots)

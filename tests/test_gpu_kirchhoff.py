@@ -39,6 +39,23 @@ def test_eikonal_matches_oracle_bit_for_bit(dx, sz, sx, iMax):
     _exact(sssi_amd.eikonal2d(V, dx, sz, sx, iMax), oracle.eikonal2d(V, dx, sz, sx, iMax))
 
 
+@pytest.mark.parametrize("layout", ["plain", "4x32", "4x64", "8x48", "8x64"])
+def test_eikonal_pass_layouts_match_oracle(layout, monkeypatch):
+    """Every pass kernel (one pass per launch, or H passes per launch on
+    overlapping tiles) gives the oracle's tables bit for bit: tiles in both
+    directions, pass counts that are and are not multiples of H, sources on
+    edges, corners and tile seams."""
+    monkeypatch.setenv("SSSI_EIK", layout)
+    V = _model(131, 157, seed=13)
+    sz = [1, 131, 48, 49, 97, 1]
+    sx = [1, 157, 48, 49, 2, 90]
+    for iMax in (9, 10):
+        print(f"layout {layout} iMax={iMax}", flush=True)
+        T = sssi_amd.eikonal2d_batch(V, 10.0, sz, sx, iMax)
+        for k in range(len(sz)):
+            _exact(T[:, :, k], oracle.eikonal2d(V, 10.0, sz[k], sx[k], iMax))
+
+
 def test_eikonal_batch_is_each_source_alone():
     V = _model(33, 47, seed=5)
     sz = [1, 1, 33, 12, 20]
@@ -90,6 +107,17 @@ def test_handle_tables_and_stack_match_app_loop():
     _exact(stack, ref)
     # the stack alone is the same call without images
     _exact(k.migrate_shots(xRec, shots, ixs), ref)
+    # resident records, a shot count that is not a multiple of the kernel's shot batch
+    k.load_shots(shots[:, :, :7])
+    ref7 = np.zeros((nz, nx))
+    for s in range(7):
+        ref7 = ref7 + images[:, :, s]
+    st7, im7 = k.migrate_loaded(xRec, ixs[:7], images=True)
+    _exact(im7, images[:, :, :7])
+    _exact(st7, ref7)
+    _exact(k.migrate_loaded(xRec[::3], ixs[:7]),
+           sum(oracle.kirchhoff_migrate(tt, xRec[::3], xsr, shots[:, :, s], dt, nz, int(ixs[s]))
+               for s in range(7)))
     k.close()
 
 
