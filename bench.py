@@ -354,9 +354,22 @@ def bench_copy(args):
     wall = time.perf_counter() - t0
     hip.hipFree(a)
     hip.hipFree(b)
+    # the achievable streaming rate for a stencil step's minimal access pattern
+    # (3 FP32 reads + 1 write per cell) at the 2-D headline grid's size (MALL
+    # resident), the 3-D grid's and a 4x larger one
+    import sssi_amd
+    from sssi_amd._lib import _call
+    stream = {}
+    for label, cells in (("2d_1040x3200", 1040 * 3200), ("3d_420x440x440", 420 * 440 * 440),
+                         ("4x_3d", 4 * 420 * 440 * 440)):
+        us, gbs = ctypes.c_double(), ctypes.c_double()
+        sssi_amd.lib()
+        _call("sssi_calib_stream", cells - cells % 4, 50, ctypes.byref(us), ctypes.byref(gbs))
+        stream[label] = {"cells": cells, "us": round(us.value, 2), "GB_per_s": round(gbs.value, 1),
+                         "frac_of_8TBs": round(gbs.value / HBM_PEAK_GBS, 4)}
     return {"metric": "device copy GB/s (calibration)", "value": round(2 * n * args.steps / wall / 1e9, 1),
             "unit": "GB/s", "bytes_read_per_copy": n, "bytes_written_per_copy": n,
-            "steps": args.steps}
+            "steps": args.steps, "stream_triad": stream}
 
 
 def bench_kirchhoff(args):

@@ -324,6 +324,14 @@ int sssi_kirchhoff_timing(sssi_kirchhoff *h, double *eikonal_ms, int64_t *passes
 void sssi_kirchhoff_destroy(sssi_kirchhoff *h);
 
 /* ---------------------------------------------------------------------
+ * Calibration (no counterpart in the reference): the device's achievable
+ * streaming rate for the access pattern of a stencil step at its minimum,
+ * three FP32 arrays of ncells read and one written once per launch
+ * (16 B per cell), averaged over reps launches with HIP events.
+ * ------------------------------------------------------------------- */
+int sssi_calib_stream(int64_t ncells, int reps, double *us_per_launch, double *gb_per_s);
+
+/* ---------------------------------------------------------------------
  * MAT-file (Level 5) I/O of real double arrays: the durable format of the
  * results above, for callers without MATLAB.  sssi_mat_write writes nvar
  * variables (names[v], data[v] column-major, ndims[v] dimensions each, dims

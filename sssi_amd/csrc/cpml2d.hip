@@ -225,6 +225,49 @@ __device__ __forceinline__ void interior_lap(const Job2D &J, const float (&win)[
     pin(lap);
 }
 
+// The same Laplacian with the x neighbourhood exchanged inside the wavefront
+// instead of through LDS: a chain of DPP wave shifts by one lane (wave_shr:1
+// / wave_shl:1) walks each row's value m = 1 .. L columns left and right, and
+// the lane with no source lane (lane 0 / lane 63) keeps the DPP's old operand,
+// the x-halo value of that column broadcast from the lane of the halo
+// registers that loaded it (readlane).  Same operands in the same FMA order
+// as interior_lap, so the results are identical bit for bit.
+__device__ __forceinline__ float shr1_in(float v, float from, int src_lane) {
+    const int h = __builtin_amdgcn_readlane(__float_as_int(from), src_lane);
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(h, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float shl1_in(float v, float from, int src_lane) {
+    const int h = __builtin_amdgcn_readlane(__float_as_int(from), src_lane);
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(h, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+template <int N, int RZ, int W, int HPL>
+__device__ __forceinline__ void interior_lap_dpp(const Job2D &J, const float (&win)[W], int OFF,
+                                                 const float (&hal)[HPL], float (&lap)[RZ]) {
+    constexpr int L = 2 * N - 1;
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+        const float v = win[OFF + L + r];
+        float acc = J.w0 * v;
+#pragma unroll
+        for (int m = 1; m <= L; ++m)
+            acc = fmaf(J.wz[m], win[OFF + L + r + m] + win[OFF + L + r - m], acc);
+        float sl = v, sr = v;
+#pragma unroll
+        for (int m = 1; m <= L; ++m) {
+            // halo element e = lane + 64u holds row e / 2L, slot e % 2L: slot L-m is
+            // column x0-m, slot L+m-1 column x0+63+m
+            const int el = r * 2 * L + L - m, er = r * 2 * L + L + m - 1;
+            sl = shr1_in(sl, hal[el / 64], el % 64);
+            sr = shl1_in(sr, hal[er / 64], er % 64);
+            acc = fmaf(J.wx[m], sr + sl, acc);
+        }
+        lap[r] = fmaf(J.w0lo, v, acc);
+    }
+    pin(lap);
+}
+
 // x halo of RZ rows (L columns each side of the 64-wide tile): lane e of
 // load u fetches value e + 64u; h_lds is its LDS slot, or for lanes without a
 // value the lane's word of the wave's spare LDS row (index dummy + lane), so
@@ -503,8 +546,11 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
 // computes.  Arithmetic is that of the tile path's interior branch.  The 4
 // wavefronts of a block march the same 64 columns, stacked in z, so the rows
 // their windows share are read by one CU.
-template <int N, int RZ, int NG>
+// NG = groups, plus 10 for the DPP x exchange (interior_lap_dpp, no LDS).
+template <int N, int RZ, int NGC>
 __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, float *lds_p) {
+    constexpr int NG = NGC % 10;
+    constexpr bool XD = NGC >= 10;
     constexpr int L = 2 * N - 1;
     constexpr int PW = kTX + 2 * L;
     constexpr int HPL = (2 * L * RZ + 63) / 64;
@@ -568,13 +614,17 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
         __builtin_amdgcn_sched_barrier(0);
         if (g + 1 < NG) issue(g + 1);
         const int zs = z0 + g * RZ;
-#pragma unroll
-        for (int r = 0; r < RZ; ++r) sp[r * PW + L + lane] = win[g * RZ + L + r];
-#pragma unroll
-        for (int u = 0; u < HPL; ++u) sp[h_lds[u]] = hal[g][u];
-        wave_lds_sync();
         float lap[RZ], s[RZ], pc[RZ], pog[RZ], v2g[RZ];
-        interior_lap<N, RZ>(J, win, g * RZ, sp, lane, lap);
+        if constexpr (XD) {
+            interior_lap_dpp<N, RZ>(J, win, g * RZ, hal[g], lap);
+        } else {
+#pragma unroll
+            for (int r = 0; r < RZ; ++r) sp[r * PW + L + lane] = win[g * RZ + L + r];
+#pragma unroll
+            for (int u = 0; u < HPL; ++u) sp[h_lds[u]] = hal[g][u];
+            wave_lds_sync();
+            interior_lap<N, RZ>(J, win, g * RZ, sp, lane, lap);
+        }
         tile_sources<RZ>(J, S.t, zs, txi, j, true, s);
 #pragma unroll
         for (int r = 0; r < RZ; ++r) {
@@ -973,6 +1023,7 @@ void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector
         SSSI_NRZ_CASES(1, CALL) SSSI_NRZ_CASES(2, CALL) SSSI_NRZ_CASES(3, CALL)           \
         SSSI_NRZ_CASES(4, CALL) SSSI_NRZ_CASES(5, CALL) SSSI_NRZ_CASES(6, CALL)           \
         SSSI_NRZ_CASES(7, CALL) SSSI_NRZ_CASES(8, CALL)                                   \
+        case 5 * 100 + 4 * 10 + 13: CALL(5, 4, 13); break;                                 \
         default: fail(SSSI_EINVAL, "no 2-D kernel for nDiffOrder=%d rz=%d groups=%d", (N), \
                       (RZ), (NG));                                                        \
     }
@@ -987,9 +1038,17 @@ void launch_cpml2d_single(int N, int rz, int ng, const Job2D &job, const int2 *o
     SSSI_HIP(hipGetLastError());
 }
 
+// SSSI_XDPP=1: the interior march exchanges x neighbours by DPP (built for
+// nDiffOrder 5, 4-row tiles, 3 groups: the benchmark layout).
+static bool xdpp_enabled(int N, int rz, int ng) {
+    const char *e = std::getenv("SSSI_XDPP");
+    return e && std::atoi(e) != 0 && N == 5 && rz == 4 && ng == 3;
+}
+
 void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
                           int step, hipStream_t s, const int *base) {
-    launch_cpml2d_single(g.N, g.rz, g.ng, job, order, nblk, step, s, base);
+    launch_cpml2d_single(g.N, g.rz, g.ng + (xdpp_enabled(g.N, g.rz, g.ng) ? 10 : 0), job, order,
+                         nblk, step, s, base);
 }
 
 void launch_cpml2d_batch(int N, int rz, int ng, const Job2D *jobs, const int2 *order, int nblk,

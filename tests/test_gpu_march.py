@@ -68,6 +68,35 @@ def test_forward_march_equals_tile_path(N):
     assert np.linalg.norm(d0 - d_ref) / np.linalg.norm(d_ref) < 1e-5
 
 
+def test_dpp_x_exchange_equals_lds_exchange():
+    """SSSI_XDPP=1 (nDiffOrder 5, 4-row tiles, 3 groups): the interior march
+    takes its x neighbours by DPP wave shifts instead of LDS -- the same
+    operands in the same order, so the same bits, forward and reverse."""
+    nz, nx, nB, N, nt = 230, 610, 20, 5, 200
+    v = model(nz, nx, 35)
+    dt = 4e-4
+    w = synth.ricker(25, nt, dt)
+    iz, ix = [0, 97, 150], [305, 333, 600]
+    ww = np.stack([w * (1 + 0.1 * k) for k in range(3)], axis=1)
+    data = np.zeros((nx, nt), order="F")
+    data[:, :] = w[None, :] * np.linspace(0.5, 1.5, nx)[:, None]
+
+    def run():
+        f = sssi_amd.fwd_cpml_2d_points(v, nt, N, nB, 5.0, 5.0, dt, iz, ix, ww,
+                                        want_snapshot=True, snap_stride=13)
+        r = sssi_amd.rvs_cpml_2d_ex(v, data, N, nB, 5.0, 5.0, dt, want_snapshot=True,
+                                    snap_stride=17)
+        return f + r
+    lds = with_layout(3, 4, run)
+    os.environ["SSSI_XDPP"] = "1"
+    try:
+        dpp = with_layout(3, 4, run)
+    finally:
+        os.environ.pop("SSSI_XDPP", None)
+    for a, b in zip(lds, dpp):
+        assert np.array_equal(a, b)
+
+
 def test_reverse_march_equals_tile_path():
     nz, nx, nB, N, nt = 200, 450, 15, 4, 240
     v = model(nz, nx, 41)
