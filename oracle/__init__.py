@@ -68,6 +68,15 @@ def lib():
                                             ctypes.c_int, _D]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_freq_cpml_2d_nnz.restype = _i64
+        L.orc_freq_cpml_2d_nnz.argtypes = [_i64, _i64, ctypes.c_int]
+        _P64 = ctypes.POINTER(ctypes.c_int64)
+        L.orc_freq_cpml_2d_matrix.argtypes = [_D, _i64, _i64, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              _P64, _P64, _D]
+        L.orc_freq_cpml_2d_solve.argtypes = [_D, _i64, _i64, ctypes.c_double, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_double, ctypes.c_double, _D,
+                                             ctypes.c_int, _D]
         _lib = L
     return _lib
 
@@ -279,3 +288,37 @@ def kirchhoff_migrate(travelTime, xRecGrid, xShotAndRecGrid, shot, dt, nz, ixs):
                                        _dp(shot), shot.shape[0], float(dt), int(ixs), _dp(m)),
            "kirchhoff_migrate")
     return m
+
+
+def freq_cpml_2d_matrix(model, w, N, nB, dz, dx):
+    """A of freqCpmlFor2dAw (src/freqCpmlFor2dAw.m:43-123) as CSR arrays
+    (rowptr, col, val complex128); model is the squared slowness (nz, nx)."""
+    m = _f64(model)
+    nz, nx = m.shape
+    nnz = lib().orc_freq_cpml_2d_nnz(nz, nx, int(N))
+    rp = np.zeros(nz * nx + 1, dtype=np.int64)
+    col = np.zeros(nnz, dtype=np.int64)
+    val = np.zeros(nnz, dtype=np.complex128)
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    _check(lib().orc_freq_cpml_2d_matrix(_dp(m), nz, nx, float(w), int(N), int(nB), float(dz),
+                                         float(dx), rp.ctypes.data_as(P64),
+                                         col.ctypes.data_as(P64),
+                                         val.view(np.float64).ctypes.data_as(_D)),
+           "freq_cpml_2d_matrix")
+    return rp, col, val
+
+
+def freq_cpml_2d_solve(model, source, w, N, nB, dz, dx):
+    """U = A \\ (-S) (src/freqCpmlFor2dAw.m:127-133) by banded LU with partial
+    pivoting; source (nz*nx, nsrc) or (nz, nx[, nsrc]) complex."""
+    m = _f64(model)
+    nz, nx = m.shape
+    S = np.asarray(source, dtype=np.complex128).reshape(nz * nx, -1, order="F")
+    ns = S.shape[1]
+    s1 = np.ascontiguousarray(S.ravel(order="F"))
+    u1 = np.zeros(nz * nx * ns, dtype=np.complex128)
+    _check(lib().orc_freq_cpml_2d_solve(_dp(m), nz, nx, float(w), int(N), int(nB), float(dz),
+                                        float(dx), s1.view(np.float64).ctypes.data_as(_D), ns,
+                                        u1.view(np.float64).ctypes.data_as(_D)),
+           "freq_cpml_2d_solve")
+    return u1.reshape(nz * nx, ns, order="F")

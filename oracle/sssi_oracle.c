@@ -907,3 +907,243 @@ int orc_kirchhoff_migrate(const double *tt, int64_t nz, int64_t nx, int64_t ntab
     free(ridx);
     return bad ? ORC_EINVAL : 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Frequency-domain CPML Helmholtz: src/freqCpmlFor2dAw.m:43-133.
+ *
+ * Matrix (:43-123).  model is the squared slowness (nz, nx); v = sqrt(1/m).
+ * Damping with dampPml (src/dampPml.m:16-18, d = d0 .* (u/L).^2, the square
+ * taken first): x on the first nB columns with u = (nB-j) dx, the last nB with
+ * u = (j+1) dx, z on the last nB rows with u = (i+1) dz, L = nB dx / nB dz;
+ * no top layer.  Summed coefficients c(-k..k) of dCoef (k = 2N-1) in the
+ * reference's loop order (:80-89).  Row (i, j) holds, for o = -k..-1, 1..k
+ * (neighbours outside the grid dropped, :120-122):
+ *   (i, j+o)  c(o) * (1j w / (dx (1j w + dx_ij)))^2
+ *   (i+o, j)  c(o) * (1j w / (dz (1j w + dz_ij)))^2
+ *   (i, j)    (m w^2 + c(0) (..dz..)^2) + c(0) (..dx..)^2
+ * always with the row cell's own damping.  The complex factor is computed as
+ *   t = (s d, s w);  q = (0, w) / t = (w t.im / |t|^2, w t.re / |t|^2);
+ *   q^2 = (q.re q.re - q.im q.im, q.re q.im + q.im q.re),
+ * each operation rounded separately (MATLAB's own complex division and power
+ * are not reproducible at the last bit: parity unpinned there, pinned by the
+ * closed form to rounding).  CSR columns ascend within a row; the index of
+ * cell (i, j) is j nz + i, as MATLAB's column-major reshape.
+ *
+ * Solve (:127-133, snapshot = A \ (-s)).  MATLAB's sparse mldivide is not
+ * restated; the solution is that of the banded LU with partial pivoting of
+ * LAPACK's zgbtf2/zgbtrs, in the natural ordering (bandwidth kl = ku = k nz):
+ * pivot = first row of maximal |re| + |im| among j..j+kl, rows swapped over
+ * columns j..ju (ju the running reach of U), column scaled by the reciprocal
+ * of the pivot, rank-1 update of columns j+1..ju, each element updated as
+ * a - l u in pivot order; then the forward sweep with the interchanges and
+ * the column-oriented backward sweep (b_j = b_j / u_jj by the textbook
+ * division, b_i -= u_ij b_j).  Pinned by the residual and by an independent
+ * sparse direct solver (SuperLU) in the tests. */
+typedef struct { double re, im; } orc_cx;
+
+static inline orc_cx cx_mul(orc_cx a, orc_cx b)
+{
+    orc_cx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return r;
+}
+static inline orc_cx cx_sub_mul(orc_cx a, orc_cx l, orc_cx u)  /* a - l u */
+{
+    orc_cx r = {a.re - (l.re * u.re - l.im * u.im), a.im - (l.re * u.im + l.im * u.re)};
+    return r;
+}
+static inline orc_cx cx_div(orc_cx a, orc_cx b)
+{
+    const double den = b.re * b.re + b.im * b.im;
+    orc_cx r = {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+    return r;
+}
+static inline orc_cx cx_recip(orc_cx b)
+{
+    const double den = b.re * b.re + b.im * b.im;
+    orc_cx r = {b.re / den, -b.im / den};
+    return r;
+}
+
+/* (1j w / (s (1j w + d)))^2 as specified above */
+static orc_cx freq_factor(double w, double s, double d)
+{
+    const double tr = s * d, ti = s * w;
+    const double den = tr * tr + ti * ti;
+    const double qr = w * ti / den, qi = w * tr / den;
+    orc_cx r = {qr * qr - qi * qi, qr * qi + qi * qr};
+    return r;
+}
+
+/* Damping grids xd, zd (nz x nx) and summed coefficients c[0..2k]. */
+static int freq_setup(const double *m, int64_t nz, int64_t nx, int N, int nB, double dz,
+                      double dx, double *xd, double *zd, double *c)
+{
+    double coeff[8];
+    if (orc_dcoef(N, coeff)) return ORC_EINVAL;
+    const int k = 2 * N - 1;
+    for (int o = 0; o <= 2 * k; ++o) c[o] = 0;
+    for (int ii = 1; ii <= N; ++ii)
+        for (int jj = 1; jj <= N; ++jj) {
+            const int o1 = ii + jj - 1, o2 = ii - jj;
+            const double p = coeff[ii - 1] * coeff[jj - 1];
+            c[o1 + k] = c[o1 + k] + p;
+            c[o2 + k] = c[o2 + k] - p;
+            c[-o1 + k] = c[-o1 + k] + p;
+            c[-o2 + k] = c[-o2 + k] - p;
+        }
+    const double logR = log(1e-6);
+    for (int64_t q = 0; q < nz * nx; ++q) { xd[q] = 0; zd[q] = 0; }
+    if (nB == 0) return 0;
+    const double Lx = nB * dx, Lz = nB * dz;
+    for (int j = 0; j < nB; ++j)
+        for (int64_t i = 0; i < nz; ++i) {
+            const int64_t jl = j, jr = nx - nB + j;
+            const double ul = (nB - j) * dx, ur = (j + 1) * dx;
+            const double vl = sqrt(1. / m[jl * nz + i]), vr = sqrt(1. / m[jr * nz + i]);
+            const double d0l = -(3 * vl) / (2 * Lx) * logR, d0r = -(3 * vr) / (2 * Lx) * logR;
+            const double sl = (ul / Lx) * (ul / Lx), sr = (ur / Lx) * (ur / Lx);
+            xd[jl * nz + i] = d0l * sl;
+            xd[jr * nz + i] = d0r * sr;
+        }
+    for (int64_t j = 0; j < nx; ++j)
+        for (int i = 0; i < nB; ++i) {
+            const int64_t ii = nz - nB + i;
+            const double u = (i + 1) * dz;
+            const double v = sqrt(1. / m[j * nz + ii]);
+            const double d0 = -(3 * v) / (2 * Lz) * logR;
+            zd[j * nz + ii] = d0 * ((u / Lz) * (u / Lz));
+        }
+    return 0;
+}
+
+int64_t orc_freq_cpml_2d_nnz(int64_t nz, int64_t nx, int N)
+{
+    const int k = 2 * N - 1;
+    int64_t nnz = 0;
+    for (int64_t i = 0; i < nz; ++i) {
+        int64_t zc = 0;
+        for (int o = -k; o <= k; ++o) zc += (o != 0 && i + o >= 0 && i + o < nz);
+        nnz += zc * nx;
+    }
+    for (int64_t j = 0; j < nx; ++j) {
+        int64_t xc = 0;
+        for (int o = -k; o <= k; ++o) xc += (o != 0 && j + o >= 0 && j + o < nx);
+        nnz += xc * nz;
+    }
+    return nnz + nz * nx;
+}
+
+int orc_freq_cpml_2d_matrix(const double *m, int64_t nz, int64_t nx, double w, int N, int nB,
+                            double dz, double dx, int64_t *rowptr, int64_t *col, double *val)
+{
+    if (!m || !rowptr || !col || !val || nz < 1 || nx < 1 || N < 1 || N > 8 || nB < 0 ||
+        2 * (int64_t)nB > nx || nB > nz)
+        return ORC_EINVAL;
+    const int k = 2 * N - 1;
+    const int64_t n = nz * nx;
+    double c[31];
+    double *xd = malloc(n * sizeof(double)), *zd = malloc(n * sizeof(double));
+    if (!xd || !zd) { free(xd); free(zd); return ORC_ENOMEM; }
+    if (freq_setup(m, nz, nx, N, nB, dz, dx, xd, zd, c)) { free(xd); free(zd); return ORC_EINVAL; }
+    int64_t e = 0;
+    for (int64_t r = 0; r < n; ++r) {
+        const int64_t j = r / nz, i = r - j * nz;
+        const orc_cx fz = freq_factor(w, dz, zd[r]), fx = freq_factor(w, dx, xd[r]);
+        rowptr[r] = e;
+        /* ascending column order: x neighbours left, z neighbours, centre, z, x right */
+        for (int o = -k; o <= -1; ++o)
+            if (j + o >= 0) { col[e] = r + o * nz; val[2 * e] = c[o + k] * fx.re; val[2 * e + 1] = c[o + k] * fx.im; ++e; }
+        for (int o = -k; o <= -1; ++o)
+            if (i + o >= 0) { col[e] = r + o; val[2 * e] = c[o + k] * fz.re; val[2 * e + 1] = c[o + k] * fz.im; ++e; }
+        {
+            const double mw = m[r] * (w * w);
+            col[e] = r;
+            val[2 * e] = (mw + c[k] * fz.re) + c[k] * fx.re;
+            val[2 * e + 1] = c[k] * fz.im + c[k] * fx.im;  /* m w^2 is real */
+            ++e;
+        }
+        for (int o = 1; o <= k; ++o)
+            if (i + o < nz) { col[e] = r + o; val[2 * e] = c[o + k] * fz.re; val[2 * e + 1] = c[o + k] * fz.im; ++e; }
+        for (int o = 1; o <= k; ++o)
+            if (j + o < nx) { col[e] = r + o * nz; val[2 * e] = c[o + k] * fx.re; val[2 * e + 1] = c[o + k] * fx.im; ++e; }
+    }
+    rowptr[n] = e;
+    free(xd);
+    free(zd);
+    return 0;
+}
+
+/* A U = -S for nsrc right-hand sides (S, U complex interleaved (n, nsrc)). */
+int orc_freq_cpml_2d_solve(const double *m, int64_t nz, int64_t nx, double w, int N, int nB,
+                           double dz, double dx, const double *S, int nsrc, double *U)
+{
+    if (!S || !U || nsrc < 1) return ORC_EINVAL;
+    const int64_t n = nz * nx, nnz = orc_freq_cpml_2d_nnz(nz, nx, N);
+    const int64_t kl = (int64_t)(2 * N - 1) * nz, ku = kl, ldab = 2 * kl + ku + 1;
+    int64_t *rp = malloc((n + 1) * sizeof(int64_t)), *ci = malloc(nnz * sizeof(int64_t));
+    double *va = malloc(2 * nnz * sizeof(double));
+    orc_cx *ab = calloc((size_t)ldab * n, sizeof(orc_cx));
+    int64_t *piv = malloc(n * sizeof(int64_t));
+    int rc = 0;
+    if (!rp || !ci || !va || !ab || !piv) { rc = ORC_ENOMEM; goto done; }
+    if ((rc = orc_freq_cpml_2d_matrix(m, nz, nx, w, N, nB, dz, dx, rp, ci, va))) goto done;
+    /* band storage: A(i, j) at ab[(kl + ku + i - j) + j ldab] */
+    for (int64_t r = 0; r < n; ++r)
+        for (int64_t e = rp[r]; e < rp[r + 1]; ++e) {
+            orc_cx v = {va[2 * e], va[2 * e + 1]};
+            ab[(kl + ku + r - ci[e]) + ci[e] * ldab] = v;
+        }
+#define AB(i, j) ab[(kl + ku + (i) - (j)) + (j) * ldab]
+    int64_t ju = 0;
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t km = kl < n - 1 - j ? kl : n - 1 - j;
+        int64_t jp = 0;
+        double best = -1;
+        for (int64_t t = 0; t <= km; ++t) {
+            const orc_cx a = AB(j + t, j);
+            const double s = fabs(a.re) + fabs(a.im);
+            if (s > best) { best = s; jp = t; }
+        }
+        piv[j] = j + jp;
+        const int64_t reach = j + ku + jp < n - 1 ? j + ku + jp : n - 1;
+        if (reach > ju) ju = reach;
+        if (best == 0) { rc = ORC_EINVAL; goto done; }  /* singular */
+        if (jp != 0)
+            for (int64_t cc = j; cc <= ju; ++cc) {
+                orc_cx t = AB(j, cc);
+                AB(j, cc) = AB(j + jp, cc);
+                AB(j + jp, cc) = t;
+            }
+        if (km > 0) {
+            const orc_cx r = cx_recip(AB(j, j));
+            for (int64_t t = 1; t <= km; ++t) AB(j + t, j) = cx_mul(AB(j + t, j), r);
+            #pragma omp parallel for schedule(static) if (km * (ju - j) > 65536)
+            for (int64_t cc = j + 1; cc <= ju; ++cc) {
+                const orc_cx u = AB(j, cc);
+                for (int64_t t = 1; t <= km; ++t) AB(j + t, cc) = cx_sub_mul(AB(j + t, cc), AB(j + t, j), u);
+            }
+        }
+    }
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 0; s < nsrc; ++s) {
+        orc_cx *b = (orc_cx *)(U + 2 * (int64_t)s * n);
+        for (int64_t q = 0; q < n; ++q) { b[q].re = -S[2 * ((int64_t)s * n + q)]; b[q].im = -S[2 * ((int64_t)s * n + q) + 1]; }
+        for (int64_t j = 0; j < n - 1; ++j) {
+            const int64_t lm = kl < n - 1 - j ? kl : n - 1 - j;
+            const int64_t l = piv[j];
+            if (l != j) { orc_cx t = b[l]; b[l] = b[j]; b[j] = t; }
+            const orc_cx bj = b[j];
+            for (int64_t t = 1; t <= lm; ++t) b[j + t] = cx_sub_mul(b[j + t], AB(j + t, j), bj);
+        }
+        for (int64_t j = n - 1; j >= 0; --j) {
+            b[j] = cx_div(b[j], AB(j, j));
+            const orc_cx bj = b[j];
+            const int64_t i0 = j - kl - ku > 0 ? j - kl - ku : 0;
+            for (int64_t i = i0; i < j; ++i) b[i] = cx_sub_mul(b[i], AB(i, j), bj);
+        }
+    }
+#undef AB
+done:
+    free(rp); free(ci); free(va); free(ab); free(piv);
+    return rc;
+}

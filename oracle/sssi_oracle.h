@@ -101,6 +101,16 @@ int orc_kirchhoff_migrate(const double *tt, int64_t nz, int64_t nx, int64_t ntab
                           const int *xrec, int nrec, const int *xsr, int nsr,
                           const double *shot, int64_t nt, double dt, int ixs, double *m);
 
+/* freqCpmlFor2dAw (src/freqCpmlFor2dAw.m): the impedance matrix A in CSR
+ * (rowptr[n+1], col[nnz], val[2 nnz] complex interleaved; nnz from
+ * orc_freq_cpml_2d_nnz) for squared slowness m (nz, nx), angular frequency w;
+ * and U = A \ (-S) for nsrc complex right-hand sides S (n, nsrc). */
+int64_t orc_freq_cpml_2d_nnz(int64_t nz, int64_t nx, int N);
+int orc_freq_cpml_2d_matrix(const double *m, int64_t nz, int64_t nx, double w, int N, int nB,
+                            double dz, double dx, int64_t *rowptr, int64_t *col, double *val);
+int orc_freq_cpml_2d_solve(const double *m, int64_t nz, int64_t nx, double w, int N, int nB,
+                           double dz, double dx, const double *S, int nsrc, double *U);
+
 #ifdef __cplusplus
 }
 #endif
