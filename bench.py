@@ -24,6 +24,8 @@ Sub-objects of the same line, each with its own fixed step counts:
   kirchhoff   Kirchhoff path in the reference app's geometry: eikonal
               traveltime tables for every surface position (Gcell-updates/s,
               roofline of the pass kernel) and migrated shots/hour
+  freq        frequency-domain CPML Helmholtz, one frequency of the FWI app's
+              geometry (all shots): factorisation + solve
 `--workload` runs one of them alone (and the sponge / copy calibrations).
 """
 import argparse
@@ -456,6 +458,54 @@ def bench_kirchhoff(args):
     return out
 
 
+def bench_freq(args):
+    """Frequency-domain CPML Helmholtz: one frequency of the FWI app's
+    geometry (120 x 140, N 2, 100 shots), factorisation + solve on the device."""
+    import oracle
+    import sssi_amd
+    from sssi_amd import helmholtz, workloads
+    wl = workloads.freq_fwi()
+    m, S = wl["m"], wl["S"]
+    nz, nx = m.shape
+    args_ = (m, S, wl["w"], wl["N"], wl["nB"], wl["d"], wl["d"])
+    helmholtz.solve(*args_)  # warm-up
+    steps = max(1, args.steps)
+    f_ms = s_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        U = helmholtz.solve(*args_)
+        a, b = helmholtz.last_timing()
+        f_ms += a
+        s_ms += b
+    wall = time.perf_counter() - t0
+    if not np.all(np.isfinite(U)):
+        raise RuntimeError("non-finite solution")
+    kl = (2 * wl["N"] - 1) * nz
+    out = {
+        "metric": "frequency-domain CPML solves per second (all shots of one frequency)",
+        "value": round(steps / wall, 3), "unit": "frequencies/s",
+        "ms_per_frequency": round(wall / steps * 1e3, 2),
+        "factor_ms": round(f_ms / steps, 2), "solve_ms": round(s_ms / steps, 2),
+        "steps": steps, "dtype": "c128",
+        "config": {"workload": f"freqCpmlFor2dAw, {nz}x{nx} (100x100 + nB 20), nDiffOrder 2, "
+                               f"{S.shape[1]} shots, 10 Hz; banded LU, bandwidth {kl}",
+                   "nz": nz, "nx": nx, "shots": int(S.shape[1]), "bandwidth": kl},
+    }
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        oracle.set_threads(cores)
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            oracle.freq_cpml_2d_solve(*args_)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": round(reps / el, 3), "unit": "frequencies/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (same banded LU, OpenMP over the update's columns and the "
+                      f"right-hand sides) on {cores} threads, {reps} frequencies ({el:.1f} s)"}
+    return out
+
+
 def _sub(fn, args, **over):
     """Run one sub-benchmark with its own fixed step counts; a failure is
     recorded instead of ending the run."""
@@ -550,7 +600,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="2d/3d: default kernel layout instead of the autotuned one")
     ap.add_argument("--workload", default="2d",
-                    choices=["2d", "3d", "sponge", "rtm", "serve", "copy", "kirchhoff"],
+                    choices=["2d", "3d", "sponge", "rtm", "serve", "copy", "kirchhoff", "freq"],
                     help="2d: the headline 2-D forward line with the sub-benchmarks (default); "
                          "the others: one workload alone")
     ap.add_argument("--nt", type=int, default=None,
@@ -566,7 +616,7 @@ def main():
               "not part of this build.", file=sys.stderr)
         sys.exit(2)
     one = {"3d": bench_3d, "sponge": bench_sponge, "rtm": bench_rtm, "serve": bench_serve,
-           "copy": bench_copy, "kirchhoff": bench_kirchhoff}
+           "copy": bench_copy, "kirchhoff": bench_kirchhoff, "freq": bench_freq}
     if args.workload in one:
         print(json.dumps(one[args.workload](args)), flush=True)
         return
@@ -578,6 +628,7 @@ def main():
         out["cfg3_rtm"] = _sub(bench_rtm, args, shots=64)
         out["cfg5_serve"] = _sub(bench_serve, args, jobs=60, rate=20.0)
         out["kirchhoff"] = _sub(bench_kirchhoff, args, steps=3)
+        out["freq"] = _sub(bench_freq, args, steps=3)
     print(json.dumps(out), flush=True)
 
 

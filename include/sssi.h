@@ -324,6 +324,32 @@ int sssi_kirchhoff_timing(sssi_kirchhoff *h, double *eikonal_ms, int64_t *passes
 void sssi_kirchhoff_destroy(sssi_kirchhoff *h);
 
 /* ---------------------------------------------------------------------
+ * Frequency-domain CPML Helmholtz (src/freqCpmlFor2dAw.m:1, called per
+ * frequency by applications/fwi_2d_acoustic/mainFwiFreqCpml2dAw.m:221 and the
+ * LSRTM/GN-FWI apps).  model is the squared slowness (nz, nx), column-major
+ * FP64; complex arrays are interleaved (re, im) FP64 pairs, (nz*nx, nsrc)
+ * column-major.  Cell (i, j) is unknown j*nz + i, as MATLAB's reshape.
+ * ------------------------------------------------------------------- */
+
+/* [~, snapshot] = freqCpmlFor2dAw(model, source, w, nDiffOrder, nBoundary,
+ * dz, dx): snapshot = A \ (-source), nsrc right-hand sides, solved on the
+ * device by banded LU with partial pivoting (bandwidth (2 nDiffOrder - 1) nz).
+ * The band must fit the solve's LDS window: (2 nDiffOrder - 1) nz up to about
+ * 5000 on gfx950 (SSSI_EINVAL beyond, with the limit in the message). */
+int sssi_freqCpmlFor2dAw(const double *model, int64_t nz, int64_t nx, const double *source,
+                         int nsrc, double w, int nDiffOrder, int nBoundary, double dz, double dx,
+                         double *snapshot);
+/* A = freqCpmlFor2dAw(...): the impedance matrix in CSR (rows ascending,
+ * columns ascending within a row): *nnz always; with rowptr non-NULL also
+ * rowptr[nz*nx+1], col[nnz] and val[2*nnz] (complex interleaved). */
+int sssi_freq_cpml_2d_matrix(const double *model, int64_t nz, int64_t nx, double w,
+                             int nDiffOrder, int nBoundary, double dz, double dx, int64_t *nnz,
+                             int64_t *rowptr, int64_t *col, double *val);
+/* Kernel time (HIP events) of the calling thread's last sssi_freqCpmlFor2dAw:
+ * factorisation and solve. */
+int sssi_freq_last_timing(double *factor_ms, double *solve_ms);
+
+/* ---------------------------------------------------------------------
  * Calibration (no counterpart in the reference): the device's achievable
  * streaming rate for the access pattern of a stencil step at its minimum,
  * three FP32 arrays of ncells read and one written once per launch

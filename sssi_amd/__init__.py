@@ -9,6 +9,8 @@ meaning and outputs follow the MATLAB wrappers they replace:
   data, snapshot = fwdTimeCpmlFor3dAw(v, source, nDiffOrder, nBoundary, dz, dx, dy, dt)
   data, snapshot = fwdTimeSpongeFor2dAw(v, source, nBoundary, dz, dx, dt)
   model, snapshot = rvsTimeSpongeFor2dAw(v, data, nBoundary, dz, dx, dt)
+  T = eikonal2d(V, dx, sz, sx, iMax);  m = migrate(travelTime, xRecGrid, ...)
+  A, snapshot = freqCpmlFor2dAw(model, source, w, nDiffOrder, nBoundary, dz, dx)
 
 Arrays are numpy arrays indexed like the MATLAB ones (``v[iz, ix]``,
 ``source[iz, ix, it]``, ``data[ix, it]``, ``snapshot[iz, ix, it]``).  As with
@@ -28,11 +30,12 @@ from .rtm import Rtm2D
 from .serve import Server
 from .matfile import mat_read, mat_write
 from .kirchhoff import eikonal2d, eikonal2d_batch, migrate, Kirchhoff2D
+from .helmholtz import freqCpmlFor2dAw
 
 __all__ = [
     "SssiError", "lib", "device_count", "fwdTimeCpmlFor2dAw", "rvsTimeCpmlFor2dAw",
     "fwdTimeCpmlFor3dAw", "fwdTimeSpongeFor2dAw", "rvsTimeSpongeFor2dAw",
     "fwd_cpml_2d_points", "rvs_cpml_2d_ex", "fwd_cpml_3d_points", "fwd_sponge_2d_points",
     "rvs_sponge_2d_ex", "Propagator", "synth", "Rtm2D", "Server", "mat_read", "mat_write",
-    "eikonal2d", "eikonal2d_batch", "migrate", "Kirchhoff2D",
+    "eikonal2d", "eikonal2d_batch", "migrate", "Kirchhoff2D", "freqCpmlFor2dAw",
 ]

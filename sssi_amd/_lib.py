@@ -80,6 +80,11 @@ SIGNATURES = {
     "sssi_kirchhoff_timing": [_vp, _D, ctypes.POINTER(_i64), ctypes.POINTER(_i64), _D],
     "sssi_kirchhoff_destroy": [_vp],
     "sssi_calib_stream": [_i64, _int, _D, _D],
+    "sssi_freqCpmlFor2dAw": [_D, _i64, _i64, _D, _int, _dbl, _int, _int, _dbl, _dbl, _D],
+    "sssi_freq_cpml_2d_matrix": [_D, _i64, _i64, _dbl, _int, _int, _dbl, _dbl,
+                                 ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                 ctypes.POINTER(_i64), _D],
+    "sssi_freq_last_timing": [_D, _D],
     "sssi_mat_write": [ctypes.c_char_p, _int, ctypes.POINTER(ctypes.c_char_p),
                        ctypes.POINTER(_D), _I, ctypes.POINTER(_i64)],
     "sssi_mat_read": [ctypes.c_char_p, ctypes.c_char_p, _I, ctypes.POINTER(_i64), _D, _i64],

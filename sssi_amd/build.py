@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsssi.so")
 
 SOURCES = ["common.cpp", "tune.cpp", "abi.cpp", "cpml2d.hip", "prop2d.hip", "prop3d.hip", "rtm.hip",
-           "serve.cpp", "noalloc.hip", "matfile.cpp", "kirchhoff.hip", "calib.hip"]
+           "serve.cpp", "noalloc.hip", "matfile.cpp", "kirchhoff.hip", "calib.hip", "helmholtz.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

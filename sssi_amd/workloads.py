@@ -22,6 +22,10 @@ tests and the benchmark.
                  increasing with depth 1800-4000 m/s (10-cell blur), the app's
                  dt and nt, iMax (nz + nx) / 2; n shots (16) spread over the
                  surface with seeded synthetic records
+  freq_fwi()     frequency-domain CPML Helmholtz in the geometry of the
+                 reference's frequency FWI app: 100 x 100 model + nB 20
+                 (120 x 140 squared slowness), N 2, d 10 m, 100 shots (one per
+                 surface column of the physical model, unit source), 10 Hz
 """
 import numpy as np
 
@@ -144,3 +148,19 @@ def kirchhoff(nshots=16, seed=31):
         rng.standard_normal((nt, nx, xShot.size), dtype=np.float32).astype(np.float64))
     return dict(V=V, d=d, dt=dt, nt=nt, iMax=(nz + nx) // 2, xsr=xsr, xRec=xsr, xShot=xShot,
                 ixs=xShot, shots=shots)
+
+
+def freq_fwi(f=10.0, seed=41):
+    """applications/fwi_2d_acoustic/mainFwiFreqCpml2dAw.m:93-127, :211-221:
+    nBoundary 20 (model extended down and sideways), nDiffOrder 2, one source
+    per physical surface column, solved per frequency."""
+    nz0, nx0, nB, N, d = 100, 100, 20, 2, 10.0
+    v = synth.layered_2d(nz0, nx0, 6, 1500.0, 4500.0, seed=seed)
+    # extended down and sideways as the app's model (nz + nB, nx + 2 nB)
+    ve = np.pad(v, ((0, nB), (nB, nB)), mode="edge")
+    m = 1.0 / ve ** 2
+    nz, nx = m.shape
+    xs = np.arange(nB, nB + nx0)
+    S = np.zeros((nz * nx, xs.size), dtype=np.complex128)
+    S[xs * nz + 0, np.arange(xs.size)] = 1.0
+    return dict(m=m, S=S, w=2 * np.pi * f, N=N, nB=nB, d=d)
