@@ -58,7 +58,7 @@ struct Cpml3dArgs {
     uint32_t o_xphi_in, o_xphi_out, o_xpsi, o_xbm1;
     uint32_t o_yphi_in, o_yphi_out, o_ypsi, o_ybm1;
     const int *order;     // launch order of the blocks (strip blocks first) or null
-    const int *src_cell;  // (i*ny + k)*nx + j, sorted
+    const int *src_cell;  // (i*ny + k)*nx + j, in the caller's order (scanned per cell)
     const float *src_amp;
     int nsrc;
     float *rec;   // this step's receivers [k*nx + j] or null
