@@ -83,10 +83,8 @@ int sssi_rvsTimeCpmlFor2dAw(const double *v, int64_t nz, int64_t nx, const doubl
 /* Replaces fwdTimeCpmlFor3dAw (src/fwdTimeCpmlFor3dAw.m:1), including its
  * damping profile conventions (fwdTimeCpmlFor3dAw.m:46-70).  source dense
  * (nz,nx,ny,nt); data (nx,ny,nt); snapshot (nz,nx,ny,nt).  A dense source
- * becomes the list of its nonzero cells, at most 4096 of them (SSSI_EINVAL
- * beyond; the reference accepts any source, e.g. a plane or an initial
- * wavefield): the 3-D step scans the whole list at every cell, so the limit
- * also keeps that scan short. */
+ * becomes the list of its nonzero cells (any number: a plane or an initial
+ * wavefield too), each cell's sources summed in the caller's order. */
 int sssi_fwdTimeCpmlFor3dAw(const double *v, int64_t nz, int64_t nx, int64_t ny,
                             const double *source, int64_t nt, int nDiffOrder, int nBoundary,
                             double dz, double dx, double dy, double dt, double *data,

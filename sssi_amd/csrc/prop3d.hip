@@ -58,7 +58,13 @@ struct Cpml3dArgs {
     uint32_t o_xphi_in, o_xphi_out, o_xpsi, o_xbm1;
     uint32_t o_yphi_in, o_yphi_out, o_ypsi, o_ybm1;
     const int *order;     // launch order of the blocks (strip blocks first) or null
-    const int *src_cell;  // (i*ny + k)*nx + j, in the caller's order (scanned per cell)
+    // point sources bucketed by (plane i, row k, 64-column x tile): bucket
+    // (i*ny + k)*ntx + tx holds entries src_ptr[b] .. src_ptr[b+1]-1, in the
+    // caller's order within a bucket; entry e is cell src_cell[e] with this
+    // step's amplitude src_amp[e]
+    const int *src_ptr;
+    const int *src_zr;    // [(k*ntx + tx)*2]: first and last plane with a source (lo > hi: none)
+    const int *src_cell;  // (i*ny + k)*nx + j
     const float *src_amp;
     int nsrc;
     float *rec;   // this step's receivers [k*nx + j] or null
@@ -185,6 +191,12 @@ __device__ __forceinline__ void cpml3d_sweep_inner(const Cpml3dArgs &a, float *s
     const __amdgpu_buffer_rsrc_t rO = buf_rsrc3(a.pold, a.nbytes);
     const __amdgpu_buffer_rsrc_t rV = buf_rsrc3(a.vdt2, a.nbytes);
     const uint32_t vcol = ((uint32_t)(k + kHY) * a.pitch + (uint32_t)(j + kHX)) * 4u;
+    // planes holding a source of this wave's row and x tile (wave-uniform)
+    int s_zlo = 1, s_zhi = 0;
+    if (a.nsrc > 0 && k < ny) {
+        s_zlo = ld_uniform3(a.src_zr + ((size_t)k * a.ntx + tx) * 2);
+        s_zhi = ld_uniform3(a.src_zr + ((size_t)k * a.ntx + tx) * 2 + 1);
+    }
     auto plane = [&](int i) -> uint32_t { return (uint32_t)(i + kHZ) * a.planeb; };
     int h_lds[HPT];
     uint32_t h_voff[HPT];
@@ -266,11 +278,14 @@ __device__ __forceinline__ void cpml3d_sweep_inner(const Cpml3dArgs &a, float *s
             float pn = 0.f;
             if (own && live) {
                 float s = 0.f;
-                if (a.nsrc > 0) {
+                if (a.nsrc > 0 && i >= s_zlo && i <= s_zhi) {
+                    // the wave's bucket: one plane, one row, its 64-column tile
                     const int cell = (i * ny + k) * nx + j;
-                    for (int q = 0; q < a.nsrc; ++q) {
-                        const int cq = ld_uniform3(a.src_cell + q);
-                        const float amp = ld_uniform3(a.src_amp + q);
+                    const int bk = (i * ny + k) * a.ntx + tx;
+                    const int e0 = ld_uniform3(a.src_ptr + bk), e1 = ld_uniform3(a.src_ptr + bk + 1);
+                    for (int e = e0; e < e1; ++e) {
+                        const int cq = ld_uniform3(a.src_cell + e);
+                        const float amp = ld_uniform3(a.src_amp + e);
                         if (cq == cell) s += amp;
                     }
                 }
@@ -319,6 +334,12 @@ __device__ __forceinline__ void cpml3d_sweep_strip(const Cpml3dArgs &a, float *s
     const __amdgpu_buffer_rsrc_t rV = buf_rsrc3(a.vdt2, a.nbytes);
     const __amdgpu_buffer_rsrc_t rS = buf_rsrc3(a.pool, a.pool_bytes);
     const uint32_t vcol = ((uint32_t)(k + kHY) * a.pitch + (uint32_t)(j + kHX)) * 4u;
+    // planes holding a source of this wave's row and x tile (wave-uniform)
+    int s_zlo = 1, s_zhi = 0;
+    if (a.nsrc > 0 && k < ny) {
+        s_zlo = ld_uniform3(a.src_zr + ((size_t)k * a.ntx + tx) * 2);
+        s_zhi = ld_uniform3(a.src_zr + ((size_t)k * a.ntx + tx) * 2 + 1);
+    }
     auto plane = [&](int i) -> uint32_t { return (uint32_t)(i + kHZ) * a.planeb; };
     int h_lds[HPT];
     uint32_t h_voff[HPT];
@@ -573,11 +594,14 @@ __device__ __forceinline__ void cpml3d_sweep_strip(const Cpml3dArgs &a, float *s
             float pn = 0.f;
             if (own && live) {
                 float s = 0.f;
-                if (a.nsrc > 0) {
+                if (a.nsrc > 0 && i >= s_zlo && i <= s_zhi) {
+                    // the wave's bucket: one plane, one row, its 64-column tile
                     const int cell = (i * ny + k) * nx + j;
-                    for (int q = 0; q < a.nsrc; ++q) {
-                        const int cq = ld_uniform3(a.src_cell + q);
-                        const float amp = ld_uniform3(a.src_amp + q);
+                    const int bk = (i * ny + k) * a.ntx + tx;
+                    const int e0 = ld_uniform3(a.src_ptr + bk), e1 = ld_uniform3(a.src_ptr + bk + 1);
+                    for (int e = e0; e < e1; ++e) {
+                        const int cq = ld_uniform3(a.src_cell + e);
+                        const float amp = ld_uniform3(a.src_amp + e);
                         if (cq == cell) s += amp;
                     }
                 }
@@ -659,6 +683,7 @@ class Prop3D final : public Prop {
    public:
     int64_t pitch = 0, npy = 0, nzp = 0;
     DevBuf<float> p[2], vfield;
+    DevBuf<int> src_zr;  // per (row, x tile) source plane range
     // damping strips: one pool; b-1 tables first, then the Phi/Psi state
     DevBuf<float> pool;
     size_t o_zbm1 = 0, o_xbm1 = 0, o_ybm1 = 0, o_zphi[2] = {0, 0}, o_zpsi = 0, o_xphi[2] = {0, 0},
@@ -871,23 +896,56 @@ class Prop3D final : public Prop {
                     int64_t nt) override {
         if (n < 0 || nt < 1 || (n > 0 && (!iz || !ix || !iy || !wavelet)))
             fail(SSSI_EINVAL, "set_points: bad arguments");
-        if (n > 4096) fail(SSSI_EINVAL, "set_points: at most 4096 point sources in 3-D");
+        if ((double)n * (double)nt * 4 > 64e9)
+            fail(SSSI_ENOMEM, "set_points: %d sources x %lld steps of amplitudes exceed 64 GB", n,
+                 (long long)nt);
         for (int q = 0; q < n; ++q)
             if (iz[q] < 0 || iz[q] >= nz || ix[q] < 0 || ix[q] >= nx || iy[q] < 0 || iy[q] >= ny)
                 fail(SSSI_EINVAL, "set_points: source %d outside the grid", q);
         nsrc = n;
         src_nt = nt;
         if (n == 0) {
+            src_ptr.release();
+            src_zr.release();
             src_cell.release();
             src_amp.release();
             return;
         }
+        // bucket (i*ny + k)*ntx + tx by a stable counting sort (caller's order kept
+        // within a bucket, so a cell's sources are summed in that order)
+        const int64_t ntx = (nx + kTX - 1) / kTX, nbk = nz * ny * ntx;
+        std::vector<int> ptr(nbk + 1, 0), bk(n), perm(n);
+        for (int q = 0; q < n; ++q) {
+            bk[q] = (int)(((int64_t)iz[q] * ny + iy[q]) * ntx + ix[q] / kTX);
+            ++ptr[bk[q] + 1];
+        }
+        for (int64_t b = 0; b < nbk; ++b) ptr[b + 1] += ptr[b];
+        {
+            std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+            for (int q = 0; q < n; ++q) perm[fill[bk[q]]++] = q;
+        }
         std::vector<int> cell(n);
-        for (int q = 0; q < n; ++q) cell[q] = (int)(((int64_t)iz[q] * ny + iy[q]) * nx + ix[q]);
+        for (int e = 0; e < n; ++e) {
+            const int q = perm[e];
+            cell[e] = (int)(((int64_t)iz[q] * ny + iy[q]) * nx + ix[q]);
+        }
         std::vector<float> amp((size_t)nt * n);
-        for (size_t q = 0; q < amp.size(); ++q) amp[q] = (float)wavelet[q];
+        for (int64_t t = 0; t < nt; ++t)
+            for (int e = 0; e < n; ++e) amp[(size_t)t * n + e] = (float)wavelet[(size_t)t * n + perm[e]];
+        // per (row, x tile): the first and last plane with a source
+        std::vector<int> zr((size_t)ny * ntx * 2);
+        for (size_t r = 0; r < zr.size(); r += 2) { zr[r] = INT32_MAX; zr[r + 1] = -1; }
+        for (int q = 0; q < n; ++q) {
+            int *z = &zr[((size_t)iy[q] * ntx + ix[q] / kTX) * 2];
+            z[0] = std::min(z[0], iz[q]);
+            z[1] = std::max(z[1], iz[q]);
+        }
+        src_zr.alloc(zr.size());
+        SSSI_HIP(hipMemcpy(src_zr.p, zr.data(), src_zr.bytes(), hipMemcpyHostToDevice));
+        src_ptr.alloc(ptr.size());
         src_cell.alloc(n);
         src_amp.alloc(amp.size());
+        SSSI_HIP(hipMemcpy(src_ptr.p, ptr.data(), src_ptr.bytes(), hipMemcpyHostToDevice));
         SSSI_HIP(hipMemcpy(src_cell.p, cell.data(), src_cell.bytes(), hipMemcpyHostToDevice));
         SSSI_HIP(hipMemcpy(src_amp.p, amp.data(), src_amp.bytes(), hipMemcpyHostToDevice));
     }
@@ -982,6 +1040,8 @@ class Prop3D final : public Prop {
             a.o_ypsi = ob(o_ypsi); a.o_ybm1 = ob(o_ybm1);
         }
         if (nsrc > 0 && t < src_nt) {
+            a.src_ptr = src_ptr.p;
+            a.src_zr = src_zr.p;
             a.src_cell = src_cell.p;
             a.src_amp = src_amp.p + (size_t)t * nsrc;
             a.nsrc = nsrc;

@@ -66,6 +66,35 @@ def test_fwdTimeCpmlFor3dAw_dense_signature():
     assert rel_l2(s_gpu, s_ref) < TOL
 
 
+def test_dense_plane_and_volume_sources_beyond_4096_points():
+    """A dense source with a whole plane (5040 cells, the first step) plus a
+    buried block, and repeated cells in a point list: each cell's sources are
+    summed in the caller's order, as the oracle sums them."""
+    nz, nx, ny, nB, nt, N = 24, 72, 70, 5, 30, 2
+    v = synth.cake_3d_dipping(nz, nx, ny, 3, 1500.0, 2500.0, seed=8)
+    dt = 1e-3
+    w = synth.ricker(30, nt, dt)
+    src = np.zeros((nz, nx, ny, nt))
+    src[2, :, :, :] = w[None, None, :] * np.linspace(0.5, 1.5, nx)[:, None, None]
+    src[10:13, 30:34, 40:45, :] = -0.3 * w[None, None, None, :]
+    d_ref, s_ref = oracle.fwd_cpml_3d(v, nt, N, nB, 10.0, 10.0, 10.0, dt, source=src,
+                                      want_snapshot=True, snap_stride=7)
+    d_gpu, s_gpu = sssi_amd.fwd_cpml_3d_points(
+        v, nt, N, nB, 10.0, 10.0, 10.0, dt, *np.nonzero(np.any(src != 0, axis=3)),
+        np.ascontiguousarray(src[np.any(src != 0, axis=3)].T), want_snapshot=True, snap_stride=7)
+    assert rel_l2(d_gpu, d_ref) < TOL, rel_l2(d_gpu, d_ref)
+    assert rel_l2(s_gpu, s_ref) < TOL, rel_l2(s_gpu, s_ref)
+    d_dense, _ = sssi_amd.fwdTimeCpmlFor3dAw(v, src, N, nB, 10.0, 10.0, 10.0, dt)
+    assert np.array_equal(d_dense, d_gpu)
+    # repeated cells, interleaved with others
+    iz, ix, iy = [3, 0, 3, 7, 3], [40, 10, 40, 5, 40], [20, 22, 20, 9, 20]
+    ww = np.stack([w, -w, 0.5 * w, w, 0.25 * w], axis=1)
+    d_ref, _ = oracle.fwd_cpml_3d(v, nt, N, nB, 10.0, 10.0, 10.0, dt, src_iz=iz, src_ix=ix,
+                                  src_iy=iy, wavelet=ww)
+    d_gpu, _ = sssi_amd.fwd_cpml_3d_points(v, nt, N, nB, 10.0, 10.0, 10.0, dt, iz, ix, iy, ww)
+    assert rel_l2(d_gpu, d_ref) < TOL, rel_l2(d_gpu, d_ref)
+
+
 @pytest.mark.parametrize("nz,nx,ny,nB,N,ty,zc", [
     (24, 70, 20, 0, 3, 8, 16),   # no damping strips: every block takes the interior sweep
     (10, 40, 24, 8, 4, 8, 8),    # z split at depth 0: every chunk reaches the bottom strip
