@@ -15,3 +15,7 @@ for k = 1:numel(gateways)
         ['LDFLAGS=$LDFLAGS -Wl,-rpath,' libdir], ...
         '-outdir', here, fullfile(here, [gateways{k} '.c']));
 end
+% the frequency-domain gateway uses the interleaved complex API
+mex('-R2018a', ['-I' inc], ['-L' libdir], '-lsssi', ...
+    ['LDFLAGS=$LDFLAGS -Wl,-rpath,' libdir], ...
+    '-outdir', here, fullfile(here, 'freqCpmlFor2dAw_mex.c'));

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 BUILD = os.path.join(HERE, "_build")
 GATEWAYS = ["fwdTimeCpmlFor2dAw_mex", "rvsTimeCpmlFor2dAw_mex", "fwdTimeCpmlFor3dAw_mex",
             "fwdTimeSpongeFor2dAw_mex", "rvsTimeSpongeFor2dAw_mex", "eikonal2d_mex",
-            "migrate_mex"]
+            "migrate_mex", "freqCpmlFor2dAw_mex"]
 
 
 class MexError(Exception):
@@ -51,6 +51,16 @@ def _lib(name):
         L.mock_invoke.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p,
                                   ctypes.c_char_p]
+        L.mock_complex.restype = ctypes.c_void_p
+        L.mock_complex.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                                   ctypes.c_void_p]
+        L.mock_nnz.restype = ctypes.c_longlong
+        L.mock_nnz.argtypes = [ctypes.c_void_p]
+        L.mock_kind.restype = ctypes.c_int
+        L.mock_kind.argtypes = [ctypes.c_void_p]
+        L.mock_sparse.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
+                                  ctypes.POINTER(ctypes.c_longlong),
+                                  ctypes.POINTER(ctypes.c_double)]
         L.mock_data.restype = ctypes.POINTER(ctypes.c_double)
         L.mock_data.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                 ctypes.POINTER(ctypes.c_longlong)]
@@ -69,6 +79,13 @@ def _to_mx(L, x):
         return L.mock_struct(len(x), names, vals)
     if isinstance(x, Int32Scalar):
         return L.mock_int32()
+    if np.iscomplexobj(x):
+        c = np.asarray(x, dtype=np.complex128)
+        c = c.reshape(-1, 1) if c.ndim == 1 else c
+        c = np.atleast_2d(c)
+        flat = np.ascontiguousarray(c.ravel(order="F"))
+        dims = (ctypes.c_longlong * c.ndim)(*c.shape)
+        return L.mock_complex(c.ndim, dims, flat.ctypes.data)
     a = np.asfortranarray(np.atleast_2d(np.asarray(x, dtype=np.float64))
                           if np.ndim(x) < 2 else np.asarray(x, dtype=np.float64))
     if np.ndim(x) == 1:
@@ -82,8 +99,26 @@ def _from_mx(L, p):
     dims = (ctypes.c_longlong * 8)()
     pr = L.mock_data(p, ctypes.byref(nd), dims)
     shape = tuple(dims[k] for k in range(nd.value))
+    kind = L.mock_kind(p)
+    cplx = bool(kind & 1)
+    if kind & 2:  # sparse: compressed columns -> scipy CSC
+        import scipy.sparse as sp
+        m, ncol = shape
+        nnz = int(L.mock_nnz(p))
+        jc = np.zeros(ncol + 1, dtype=np.int64)
+        ir = np.zeros(max(1, nnz), dtype=np.int64)
+        val = np.zeros(max(1, 2 * nnz if cplx else nnz))
+        L.mock_sparse(p, jc.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                      ir.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                      val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        data = val[:2 * nnz].view(np.complex128) if cplx else val[:nnz]
+        return sp.csc_matrix((data.copy(), ir[:nnz].copy(), jc), shape=(m, ncol))
     n = int(np.prod(shape))
-    flat = np.ctypeslib.as_array(pr, shape=(n,)).copy() if n else np.zeros(0)
+    if cplx:
+        flat = (np.ctypeslib.as_array(pr, shape=(2 * n,)).copy().view(np.complex128)
+                if n else np.zeros(0, dtype=np.complex128))
+    else:
+        flat = np.ctypeslib.as_array(pr, shape=(n,)).copy() if n else np.zeros(0)
     return flat.reshape(shape, order="F")
 
 

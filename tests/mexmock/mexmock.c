@@ -20,7 +20,10 @@ struct mxArray_tag {
     int complex_flag;
     mwSize ndim;
     mwSize dims[8];
-    double *pr;
+    double *pr;     /* complex: interleaved (re, im) */
+    int sparse;     /* compressed columns: jc[n+1], ir[nzmax], nzmax values in pr */
+    mwIndex *jc, *ir;
+    mwSize nzmax;
     int nfields;
     char names[MOCK_MAX_FIELDS][32];
     mxArray *fields[MOCK_MAX_FIELDS];
@@ -40,6 +43,13 @@ static void *track(void *p)
 int mxIsDouble(const mxArray *a) { return a->cls == mxDOUBLE_CLASS; }
 int mxIsComplex(const mxArray *a) { return a->complex_flag; }
 int mxIsStruct(const mxArray *a) { return a->cls == mxSTRUCT_CLASS; }
+int mxIsSparse(const mxArray *a) { return a->sparse; }
+mxComplexDouble *mxGetComplexDoubles(const mxArray *a)
+{
+    return a->complex_flag ? (mxComplexDouble *)a->pr : NULL;
+}
+mwIndex *mxGetJc(const mxArray *a) { return a->jc; }
+mwIndex *mxGetIr(const mxArray *a) { return a->ir; }
 size_t mxGetM(const mxArray *a) { return a->dims[0]; }
 size_t mxGetN(const mxArray *a)
 {
@@ -61,7 +71,7 @@ mxArray *mxGetField(const mxArray *a, size_t index, const char *name)
     return NULL;
 }
 
-static mxArray *make_double(mwSize ndim, const mwSize *dims)
+static mxArray *make_array(mwSize ndim, const mwSize *dims, int cplx)
 {
     mxArray *a = (mxArray *)calloc(1, sizeof(mxArray));
     size_t n = 1;
@@ -73,22 +83,39 @@ static mxArray *make_double(mwSize ndim, const mwSize *dims)
         a->dims[k] = k < ndim ? dims[k] : 1;
         n *= a->dims[k];
     }
-    a->pr = (double *)calloc(n ? n : 1, sizeof(double));
+    a->complex_flag = cplx;
+    a->pr = (double *)calloc(n ? (cplx ? 2 * n : n) : 1, sizeof(double));
     return a;
 }
+
+static mxArray *make_double(mwSize ndim, const mwSize *dims) { return make_array(ndim, dims, 0); }
 
 mxArray *mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity c)
 {
     mwSize d[2] = {m, n};
-    (void)c;
-    return make_double(2, d);
+    return make_array(2, d, c == mxCOMPLEX);
 }
 
 mxArray *mxCreateNumericArray(mwSize ndim, const mwSize *dims, mxClassID cls, mxComplexity c)
 {
     (void)cls;
-    (void)c;
-    return make_double(ndim, dims);
+    return make_array(ndim, dims, c == mxCOMPLEX);
+}
+
+mxArray *mxCreateSparse(mwSize m, mwSize n, mwSize nzmax, mxComplexity c)
+{
+    mxArray *a = (mxArray *)calloc(1, sizeof(mxArray));
+    a->cls = mxDOUBLE_CLASS;
+    a->ndim = 2;
+    a->dims[0] = m;
+    a->dims[1] = n;
+    a->sparse = 1;
+    a->complex_flag = c == mxCOMPLEX;
+    a->nzmax = nzmax ? nzmax : 1;
+    a->jc = (mwIndex *)calloc(n + 1, sizeof(mwIndex));
+    a->ir = (mwIndex *)calloc(a->nzmax, sizeof(mwIndex));
+    a->pr = (double *)calloc(a->complex_flag ? 2 * a->nzmax : a->nzmax, sizeof(double));
+    return a;
 }
 
 void *mxCalloc(size_t n, size_t size) { return track(calloc(n ? n : 1, size ? size : 1)); }
@@ -125,6 +152,34 @@ mxArray *mock_double(int ndim, const long long *dims, const double *data)
     return a;
 }
 
+/* complex double array from interleaved (re, im) data */
+mxArray *mock_complex(int ndim, const long long *dims, const double *data)
+{
+    mwSize d[8];
+    size_t n = 1;
+    for (int k = 0; k < ndim && k < 8; ++k) {
+        d[k] = (mwSize)dims[k];
+        n *= d[k];
+    }
+    mxArray *a = make_array((mwSize)ndim, d, 1);
+    if (data) memcpy(a->pr, data, 2 * n * sizeof(double));
+    return a;
+}
+
+/* flags of an output: bit 0 complex, bit 1 sparse */
+int mock_kind(const mxArray *a) { return (a->complex_flag ? 1 : 0) | (a->sparse ? 2 : 0); }
+
+long long mock_nnz(const mxArray *a) { return a->sparse ? (long long)a->jc[a->dims[1]] : 0; }
+
+/* compressed columns of a sparse output: jc[n+1], ir[nnz], values (interleaved if complex) */
+void mock_sparse(const mxArray *a, long long *jc, long long *ir, double *val)
+{
+    const size_t n = a->dims[1], nnz = a->jc[n];
+    for (size_t k = 0; k <= n; ++k) jc[k] = (long long)a->jc[k];
+    for (size_t k = 0; k < nnz; ++k) ir[k] = (long long)a->ir[k];
+    memcpy(val, a->pr, (a->complex_flag ? 2 : 1) * nnz * sizeof(double));
+}
+
 mxArray *mock_int32(void)
 {
     mxArray *a = (mxArray *)calloc(1, sizeof(mxArray));
@@ -154,6 +209,8 @@ void mock_free(mxArray *a)
     if (a->cls == mxSTRUCT_CLASS)
         for (int k = 0; k < a->nfields; ++k) mock_free(a->fields[k]);
     free(a->pr);
+    free(a->jc);
+    free(a->ir);
     free(a);
 }
 

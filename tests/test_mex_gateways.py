@@ -43,7 +43,8 @@ def small_2d(nt=120):
                                         ("fwdTimeCpmlFor3dAw_mex", 8),
                                         ("fwdTimeSpongeFor2dAw_mex", 6),
                                         ("rvsTimeSpongeFor2dAw_mex", 6),
-                                        ("eikonal2d_mex", 5), ("migrate_mex", 7)])
+                                        ("eikonal2d_mex", 5), ("migrate_mex", 7),
+                                        ("freqCpmlFor2dAw_mex", 7)])
 def test_argument_count(name, nargs):
     with pytest.raises(MexError) as e:
         call(name, *([1.0] * (nargs - 1)))
@@ -236,3 +237,39 @@ def test_kirchhoff_gateways_equal_abi_and_oracle():
         call("migrate_mex", tt, xr.astype(float), xsr.astype(float), shot[:40], 1e-3, 24, 9,
              nargout=1)
     assert e.value.ident == "sssi:run"
+
+
+def test_freq_gateway_validation():
+    m = np.full((6, 8), 1.0 / 2000.0 ** 2)
+    good = [m, np.ones((48, 2)), 60.0, 2, 2, 10.0, 10.0]
+    for k, val, ident in [(1, np.ones((47, 1)), "sssi:size"), (3, 9, "sssi:range"),
+                          (3, 0, "sssi:range"), (0, np.ones((2, 3, 4)), "sssi:size"),
+                          (2, np.ones(2), "sssi:size"), (1, mexmock.Int32Scalar(), "sssi:class")]:
+        args = list(good)
+        args[k] = val
+        with pytest.raises(MexError) as e:
+            call("freqCpmlFor2dAw_mex", *args)
+        assert e.value.ident == ident
+
+
+@pytest.mark.gpu
+def test_freq_gateway_equals_abi():
+    from sssi_amd import helmholtz
+    nz, nx = 18, 22
+    m = 1.0 / synth.layered_2d(nz, nx, 3, 1500, 3000, seed=2) ** 2
+    w = 2 * np.pi * 14
+    rng = np.random.default_rng(5)
+    src = rng.standard_normal((nz, nx, 2)) + 1j * rng.standard_normal((nz, nx, 2))
+    A, snap = call("freqCpmlFor2dAw_mex", m, src, w, 3.0, 4.0, 10.0, 12.0)
+    A_ref = helmholtz.impedance_matrix(m, w, 3, 4, 10.0, 12.0)
+    assert A.shape == A_ref.shape and (A != A_ref).nnz == 0
+    assert snap.shape == src.shape
+    assert np.array_equal(snap, helmholtz.solve(m, src, w, 3, 4, 10.0, 12.0))
+    # a real source is the complex one with zero imaginary part; (n, nsrc) shape kept
+    real = src.real.reshape(nz * nx, 2, order="F")
+    _, s_real = call("freqCpmlFor2dAw_mex", m, real, w, 3.0, 4.0, 10.0, 12.0)
+    assert s_real.shape == real.shape
+    assert np.array_equal(s_real, helmholtz.solve(m, real + 0j, w, 3, 4, 10.0, 12.0))
+    # an empty source gives A and []
+    A1, s_empty = call("freqCpmlFor2dAw_mex", m, np.zeros((0, 0)), w, 3.0, 4.0, 10.0, 12.0)
+    assert (A1 != A_ref).nnz == 0 and s_empty.size == 0
