@@ -268,6 +268,37 @@ __device__ __forceinline__ void interior_lap_dpp(const Job2D &J, const float (&w
     pin(lap);
 }
 
+// The same Laplacian with the x neighbourhood read as loads shifted by m
+// columns (the sponge step's layout: no LDS, no halo exchange); the row zs+r
+// is at byte row offset srow + r * rowb of the padded field, whose 32 padding
+// columns each side keep every shifted load in bounds.  Same operands in the
+// same FMA order as interior_lap.
+template <int N, int RZ, int W>
+__device__ __forceinline__ void interior_lap_glb(const Job2D &J, const float (&win)[W], int OFF,
+                                                 __amdgpu_buffer_rsrc_t rP, uint32_t vcol,
+                                                 uint32_t srow, uint32_t rowb,
+                                                 float (&lap)[RZ]) {
+    constexpr int L = 2 * N - 1;
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+        const uint32_t ro = srow + r * rowb;
+        float xl[L + 1], xr[L + 1];
+#pragma unroll
+        for (int m = 1; m <= L; ++m) {
+            xl[m] = buf_ld(rP, vcol - 4u * m, ro);
+            xr[m] = buf_ld(rP, vcol + 4u * m, ro);
+        }
+        float acc = J.w0 * win[OFF + L + r];
+#pragma unroll
+        for (int m = 1; m <= L; ++m)
+            acc = fmaf(J.wz[m], win[OFF + L + r + m] + win[OFF + L + r - m], acc);
+#pragma unroll
+        for (int m = 1; m <= L; ++m) acc = fmaf(J.wx[m], xr[m] + xl[m], acc);
+        lap[r] = fmaf(J.w0lo, win[OFF + L + r], acc);
+    }
+    pin(lap);
+}
+
 // x halo of RZ rows (L columns each side of the 64-wide tile): lane e of
 // load u fetches value e + 64u; h_lds is its LDS slot, or for lanes without a
 // value the lane's word of the wave's spare LDS row (index dummy + lane), so
@@ -546,11 +577,12 @@ __device__ __forceinline__ void cpml2d_tile(const Job2D &J, int tile, int step, 
 // computes.  Arithmetic is that of the tile path's interior branch.  The 4
 // wavefronts of a block march the same 64 columns, stacked in z, so the rows
 // their windows share are read by one CU.
-// NG = groups, plus 10 for the DPP x exchange (interior_lap_dpp, no LDS).
+// NGC = groups, plus 10 for the DPP x exchange (interior_lap_dpp, no LDS) or
+// 20 for shifted global loads (interior_lap_glb, no LDS).
 template <int N, int RZ, int NGC>
 __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, float *lds_p) {
     constexpr int NG = NGC % 10;
-    constexpr bool XD = NGC >= 10;
+    constexpr int XMODE = NGC / 10;
     constexpr int L = 2 * N - 1;
     constexpr int PW = kTX + 2 * L;
     constexpr int HPL = (2 * L * RZ + 63) / 64;
@@ -615,8 +647,10 @@ __device__ __forceinline__ void cpml2d_march(const Job2D &J, int mb, int step, f
         if (g + 1 < NG) issue(g + 1);
         const int zs = z0 + g * RZ;
         float lap[RZ], s[RZ], pc[RZ], pog[RZ], v2g[RZ];
-        if constexpr (XD) {
+        if constexpr (XMODE == 1) {
             interior_lap_dpp<N, RZ>(J, win, g * RZ, hal[g], lap);
+        } else if constexpr (XMODE == 2) {
+            interior_lap_glb<N, RZ>(J, win, g * RZ, rP, vcol, srow0 + g * RZ * rowb, rowb, lap);
         } else {
 #pragma unroll
             for (int r = 0; r < RZ; ++r) sp[r * PW + L + lane] = win[g * RZ + L + r];
@@ -1024,6 +1058,7 @@ void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector
         SSSI_NRZ_CASES(4, CALL) SSSI_NRZ_CASES(5, CALL) SSSI_NRZ_CASES(6, CALL)           \
         SSSI_NRZ_CASES(7, CALL) SSSI_NRZ_CASES(8, CALL)                                   \
         case 5 * 100 + 4 * 10 + 13: CALL(5, 4, 13); break;                                 \
+        case 5 * 100 + 4 * 10 + 23: CALL(5, 4, 23); break;                                 \
         default: fail(SSSI_EINVAL, "no 2-D kernel for nDiffOrder=%d rz=%d groups=%d", (N), \
                       (RZ), (NG));                                                        \
     }
@@ -1038,17 +1073,19 @@ void launch_cpml2d_single(int N, int rz, int ng, const Job2D &job, const int2 *o
     SSSI_HIP(hipGetLastError());
 }
 
-// SSSI_XDPP=1: the interior march exchanges x neighbours by DPP (built for
-// nDiffOrder 5, 4-row tiles, 3 groups: the benchmark layout).
-static bool xdpp_enabled(int N, int rz, int ng) {
+// SSSI_XDPP=1: the interior march exchanges x neighbours by DPP; =2: reads
+// them as shifted global loads (built for nDiffOrder 5, 4-row tiles, 3 groups:
+// the benchmark layout).  Returns the layout code's offset (0, 10 or 20).
+static int xdpp_enabled(int N, int rz, int ng) {
     const char *e = std::getenv("SSSI_XDPP");
-    return e && std::atoi(e) != 0 && N == 5 && rz == 4 && ng == 3;
+    const int m = e ? std::atoi(e) : 0;
+    return (m == 1 || m == 2) && N == 5 && rz == 4 && ng == 3 ? 10 * m : 0;
 }
 
 void launch_cpml2d_single(const Cpml2dGrid &g, const Job2D &job, const int2 *order, int nblk,
                           int step, hipStream_t s, const int *base) {
-    launch_cpml2d_single(g.N, g.rz, g.ng + (xdpp_enabled(g.N, g.rz, g.ng) ? 10 : 0), job, order,
-                         nblk, step, s, base);
+    launch_cpml2d_single(g.N, g.rz, g.ng + xdpp_enabled(g.N, g.rz, g.ng), job, order, nblk, step,
+                         s, base);
 }
 
 void launch_cpml2d_batch(int N, int rz, int ng, const Job2D *jobs, const int2 *order, int nblk,

@@ -68,10 +68,12 @@ def test_forward_march_equals_tile_path(N):
     assert np.linalg.norm(d0 - d_ref) / np.linalg.norm(d_ref) < 1e-5
 
 
-def test_dpp_x_exchange_equals_lds_exchange():
-    """SSSI_XDPP=1 (nDiffOrder 5, 4-row tiles, 3 groups): the interior march
-    takes its x neighbours by DPP wave shifts instead of LDS -- the same
-    operands in the same order, so the same bits, forward and reverse."""
+@pytest.mark.parametrize("mode", ["1", "2"])
+def test_x_exchange_variants_equal_lds_exchange(mode):
+    """SSSI_XDPP=1 (DPP wave shifts) and =2 (shifted global loads), at
+    nDiffOrder 5, 4-row tiles, 3 groups: the interior march takes its x
+    neighbours without the LDS round trip -- the same operands in the same
+    order, so the same bits, forward and reverse."""
     nz, nx, nB, N, nt = 230, 610, 20, 5, 200
     v = model(nz, nx, 35)
     dt = 4e-4
@@ -88,12 +90,12 @@ def test_dpp_x_exchange_equals_lds_exchange():
                                     snap_stride=17)
         return f + r
     lds = with_layout(3, 4, run)
-    os.environ["SSSI_XDPP"] = "1"
+    os.environ["SSSI_XDPP"] = mode
     try:
-        dpp = with_layout(3, 4, run)
+        alt = with_layout(3, 4, run)
     finally:
         os.environ.pop("SSSI_XDPP", None)
-    for a, b in zip(lds, dpp):
+    for a, b in zip(lds, alt):
         assert np.array_equal(a, b)
 
 
