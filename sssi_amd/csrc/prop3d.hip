@@ -684,6 +684,7 @@ class Prop3D final : public Prop {
     int64_t pitch = 0, npy = 0, nzp = 0;
     DevBuf<float> p[2], vfield;
     DevBuf<int> src_zr;  // per (row, x tile) source plane range
+    int pd_override = 0;  // SSSI_PD3
     // damping strips: one pool; b-1 tables first, then the Phi/Psi state
     DevBuf<float> pool;
     size_t o_zbm1 = 0, o_xbm1 = 0, o_ybm1 = 0, o_zphi[2] = {0, 0}, o_zpsi = 0, o_xphi[2] = {0, 0},
@@ -716,6 +717,7 @@ class Prop3D final : public Prop {
             const int t = std::atoi(e);
             if (t == 4 || t == 8 || t == 16) ty = t;
         }
+        if (const char *e = std::getenv("SSSI_PD3")) pd_override = std::atoi(e);
         if (const char *e = std::getenv("SSSI_ZC3")) {
             const int z = std::atoi(e);
             if (z >= 8 && z <= 4096) zc = z;
@@ -956,8 +958,16 @@ class Prop3D final : public Prop {
 
     template <int NN, int TY>
     void launch(const Cpml3dArgs &args, int nblk) {
-        // ring depth: 4 planes where the registers allow it at occupancy 4
+        // ring depth: 4 planes where the registers allow it at occupancy 4;
+        // SSSI_PD3=2 forces 2 (a shallower ring keeps fewer planes in flight)
         constexpr int PD = TY >= 8 && NN <= 4 ? 4 : 2;
+        if constexpr (PD == 4) {
+            if (pd_override == 2) {
+                hipLaunchKernelGGL((cpml3d_step<NN, TY, 2>), dim3(nblk), dim3(64 * TY), 0, stream,
+                                   args);
+                return;
+            }
+        }
         hipLaunchKernelGGL((cpml3d_step<NN, TY, PD>), dim3(nblk), dim3(64 * TY), 0, stream, args);
     }
 
