@@ -978,12 +978,13 @@ class Prop3D final : public Prop {
 
     // Blocks whose stencil reach touches a damping strip (x, y, or a chunk of
     // the bottom z strip) go first (they run the two-stage form and carry the
-    // strip arrays' traffic); the others follow in 8 contiguous bands, block b
-    // landing on XCD b % 8.
+    // strip arrays' traffic); the others follow.  Both lists are split into 8
+    // contiguous bands, block b landing on XCD b % 8.
     void build_order(int ntx, int nty, int nzc) {
         const int nblk = ntx * nty * nzc;
         const int zs = zsplit(), na = nzA();
         std::vector<int> strip, inner;
+        std::vector<int> cat[8];  // strip blocks by which strips are in reach (x, y, z bits)
         for (int b = 0; b < nblk; ++b) {
             const int tx = b % ntx, tyi = (b / ntx) % nty, tz = b / (ntx * nty);
             const int64_t x0 = (int64_t)tx * kTX, y0 = (int64_t)tyi * ty;
@@ -993,21 +994,41 @@ class Prop3D final : public Prop {
             const bool ys = nB > 0 && (y0 - N < nB || y0 + ty + N - 1 > ny - nB);
             const bool zs_ = nB > 0 && zbeg >= zs;
             (xs || ys || zs_ ? strip : inner).push_back(b);
+            if (xs || ys || zs_) cat[(xs ? 1 : 0) | (ys ? 2 : 0) | (zs_ ? 4 : 0)].push_back(b);
         }
-        std::vector<int> ord(strip);
-        const size_t n = inner.size(), base = ord.size();
-        size_t next[8], end[8];
-        for (int x = 0; x < 8; ++x) {
-            next[x] = n * x / 8;
-            end[x] = n * (x + 1) / 8;
+        // each list split into 8 contiguous bands, block b of the launch taking
+        // the next entry of XCD b % 8's band, so that the tiles one XCD runs at
+        // the same time are neighbours and share their halo rows in its L2
+        std::vector<int> ord;
+        ord.reserve(nblk);
+        auto append_banded = [&](const std::vector<int> &list) {
+            const size_t n = list.size(), base = ord.size();
+            size_t next[8], end[8];
+            for (int x = 0; x < 8; ++x) {
+                next[x] = n * x / 8;
+                end[x] = n * (x + 1) / 8;
+            }
+            for (size_t b = base; b < base + n; ++b) {
+                int x = (int)(b % 8);
+                if (next[x] == end[x])
+                    for (int y = 0; y < 8; ++y)
+                        if (next[y] < end[y]) { x = y; break; }
+                ord.push_back(list[next[x]++]);
+            }
+        };
+        // strip blocks: SSSI_STRIPBAND3 = 0 natural order (round robin over the
+        // XCDs), 1 one banded list, 2 (default) banded per kind of strip in
+        // reach, so each XCD gets its share of every kind (they differ in cost)
+        const char *sb = std::getenv("SSSI_STRIPBAND3");
+        const int mode = sb ? std::atoi(sb) : 2;
+        if (mode == 0) {
+            ord = strip;
+        } else if (mode == 1) {
+            append_banded(strip);
+        } else {
+            for (int c = 7; c >= 1; --c) append_banded(cat[c]);
         }
-        for (size_t b = base; b < base + n; ++b) {
-            int x = (int)(b % 8);
-            if (next[x] == end[x])
-                for (int y = 0; y < 8; ++y)
-                    if (next[y] < end[y]) { x = y; break; }
-            ord.push_back(inner[next[x]++]);
-        }
+        append_banded(inner);
         SSSI_HIP(hipStreamSynchronize(stream));  // the previous order may be in use
         order.alloc(ord.size());
         SSSI_HIP(hipMemcpy(order.p, ord.data(), order.bytes(), hipMemcpyHostToDevice));
