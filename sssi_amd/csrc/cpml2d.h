@@ -96,6 +96,7 @@ class Cpml2dGrid {
     int ntz() const { return (int)((nz + tz() - 1) / tz()); }
     int nblocks() const { return ntx() * ntz(); }
     bool block_is_strip(int tile) const;
+    int strip_kind(int tile) const;  // bit 0: x strip in reach, bit 1: z strip
     // tile block covered by the interior march (not launched as a tile block)
     bool block_in_march(int tile) const;
     int n_march_blocks() const { return mntx > 0 ? (mz_rows / (4 * ng * rz)) * mntx : 0; }

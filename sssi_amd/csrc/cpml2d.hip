@@ -832,13 +832,15 @@ Cpml2dGrid::Cpml2dGrid(const double *v, int64_t nz_, int64_t nx_, int N_, int nB
     }
 }
 
-bool Cpml2dGrid::block_is_strip(int tile) const {
-    if (nB == 0) return false;
+bool Cpml2dGrid::block_is_strip(int tile) const { return strip_kind(tile) != 0; }
+
+int Cpml2dGrid::strip_kind(int tile) const {
+    if (nB == 0) return 0;
     const int tzi = tile / ntx(), txi = tile % ntx();
     const int64_t z0 = (int64_t)tzi * tz(), x0 = (int64_t)txi * kTX;
     const bool zs = z0 + tz() + N - 1 > nz - nB;
     const bool xs = x0 - N < nB || x0 + kTX + N - 1 > nx - nB;
-    return zs || xs;
+    return (xs ? 1 : 0) | (zs ? 2 : 0);
 }
 
 void Cpml2dGrid::set_layout(int rz_, int ng_) {
@@ -1026,18 +1028,28 @@ static void append_banded(std::vector<int2> &order, const std::vector<int2> &lis
     }
 }
 
+// Strip tiles first (banded per kind of strip in reach when SSSI_STRIPBAND2 is
+// 2, as one list when 1), then the interior march blocks, then the remaining
+// interior tiles, each banded over the 8 XCDs.
 void build_block_order(const std::vector<const Cpml2dGrid *> &grids, std::vector<int2> &order) {
-    std::vector<int2> march, strip, inner;
+    std::vector<int2> march, strip, inner, kind[4];
     for (int g = 0; g < (int)grids.size(); ++g) {
         const Cpml2dGrid &G = *grids[g];
         for (int b = 0; b < G.n_march_blocks(); ++b) march.push_back(make_int2(g, -1 - b));
         for (int t = 0; t < G.nblocks(); ++t) {
             if (G.block_in_march(t)) continue;
-            (G.block_is_strip(t) ? strip : inner).push_back(make_int2(g, t));
+            const int k = G.strip_kind(t);
+            (k ? strip : inner).push_back(make_int2(g, t));
+            if (k) kind[k].push_back(make_int2(g, t));
         }
     }
     order.clear();
-    append_banded(order, strip);
+    const char *e = std::getenv("SSSI_STRIPBAND2");
+    if (e && std::atoi(e) == 2) {
+        for (int k = 3; k >= 1; --k) append_banded(order, kind[k]);
+    } else {
+        append_banded(order, strip);
+    }
     append_banded(order, march);
     append_banded(order, inner);
 }
