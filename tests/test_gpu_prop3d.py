@@ -23,7 +23,7 @@ def rel_l2(a, b):
 
 @pytest.fixture
 def kernel_env():
-    saved = {k: os.environ.get(k) for k in ("SSSI_TY3", "SSSI_ZC3")}
+    saved = {k: os.environ.get(k) for k in ("SSSI_TY3", "SSSI_ZC3", "SSSI_RING3", "SSSI_PD3")}
     yield
     for k, v in saved.items():
         if v is None:
@@ -117,3 +117,34 @@ def test_fwd_cpml_3d_chunking_edge_cases(nz, nx, ny, nB, N, ty, zc, kernel_env):
                                                want_snapshot=True, snap_stride=11)
     assert rel_l2(d_gpu, d_ref) < TOL, rel_l2(d_gpu, d_ref)
     assert rel_l2(s_gpu, s_ref) < TOL, rel_l2(s_gpu, s_ref)
+
+
+@pytest.mark.parametrize("N,ty,zc,pd", [(4, 8, 8, 0), (4, 8, 12, 2), (3, 16, 12, 0), (5, 16, 8, 0),
+                                        (2, 4, 16, 0)])
+def test_plane_ring_equals_two_tile_buffer(N, ty, zc, pd, kernel_env):
+    """The opt-in LDS plane ring (halo loaded with the z window's front and
+    parked in LDS for L planes) computes the same values as the default
+    2-tile buffer, bit for bit, over strip and interior blocks, chunk seams
+    and the prefetch ring's depths."""
+    os.environ["SSSI_TY3"] = str(ty)
+    os.environ["SSSI_ZC3"] = str(zc)
+    if pd:
+        os.environ["SSSI_PD3"] = str(pd)
+    nz, nx, ny, nB, nt = 40, 90, 44, 6, 80
+    v = synth.cake_3d_dipping(nz, nx, ny, 4, 1500.0, 3000.0, seed=21 + N)
+    dx = dy = dz = 10.0
+    dt = 0.4 * dx / 3000.0 / np.sqrt(3) / sum(abs(c) for c in synth.dCoef(N))
+    w = synth.ricker(25, nt, dt)
+    iz, ix, iy = [0, 25], [45, 70], [20, 9]
+    ww = np.stack([w, -0.6 * w], axis=1)
+    out = {}
+    for ring in ("0", "1"):
+        os.environ["SSSI_RING3"] = ring
+        print(f"N={N} ty={ty} zc={zc} pd={pd} ring={ring}", flush=True)
+        out[ring] = sssi_amd.fwd_cpml_3d_points(v, nt, N, nB, dz, dx, dy, dt, iz, ix, iy, ww,
+                                                want_snapshot=True, snap_stride=13)
+    assert np.array_equal(out["0"][0], out["1"][0])
+    assert np.array_equal(out["0"][1], out["1"][1])
+    d_ref = oracle.fwd_cpml_3d(v, nt, N, nB, dz, dx, dy, dt, src_iz=iz, src_ix=ix, src_iy=iy,
+                               wavelet=ww)[0]
+    assert rel_l2(out["1"][0], d_ref) < TOL, rel_l2(out["1"][0], d_ref)
