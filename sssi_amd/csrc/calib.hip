@@ -41,8 +41,8 @@ extern "C" int sssi_calib_stream(int64_t ncells, int reps, double *us_per_launch
             fail(SSSI_EINVAL, "calib: ncells must be a multiple of 4 in [1024, 2^32]");
         const int64_t n4 = ncells / 4;
         DevBuf<float4> a(n4), b(n4), c(n4), d(n4);
-        hipStream_t s;
-        SSSI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        OwnedStream s;
+        s.create();
         const unsigned nb = (unsigned)((n4 + 255) / 256);
         for (float4 *p : {a.p, b.p, c.p, d.p})
             hipLaunchKernelGGL(fill_ones, dim3(nb), dim3(256), 0, s, p, n4);
@@ -60,7 +60,6 @@ extern "C" int sssi_calib_stream(int64_t ncells, int reps, double *us_per_launch
         SSSI_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        (void)hipStreamDestroy(s);
         const double us = ms * 1e3 / reps;
         if (us_per_launch) *us_per_launch = us;
         if (gb_per_s) *gb_per_s = 16.0 * (double)ncells / (us * 1e-6) / 1e9;

@@ -94,6 +94,21 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Non-blocking stream owned by its holder: destroyed with it, also when the
+// holder's constructor throws after creating it.  Declared before the device
+// buffers whose work it orders, it is destroyed after them.
+struct OwnedStream {
+    hipStream_t s = nullptr;
+    OwnedStream() = default;
+    OwnedStream(const OwnedStream &) = delete;
+    OwnedStream &operator=(const OwnedStream &) = delete;
+    ~OwnedStream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+    void create() { SSSI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    operator hipStream_t() const { return s; }
+};
+
 // v (column-major FP64, z fastest: v[(k*nx + j)*nz + i]) -> a padded FP32
 // device field: dst[(i + hz)*plane + (k + hy)*pitch + j + hx] = (v dt)^2 when
 // square, else v dt, each rounded once from FP64 (the arithmetic of a host

@@ -512,7 +512,7 @@ static void freq_validate(const double *m, int64_t nz, int64_t nx, double w, int
 struct FreqSystem {
     int64_t nz, nx, n, kl, ku, ldab;
     int k;
-    hipStream_t s = nullptr;
+    OwnedStream s;  // before the buffers: destroyed after them
     DevBuf<cx> ab;
     DevBuf<int64_t> piv, juv;
     DevBuf<LuState> st;
@@ -529,7 +529,7 @@ struct FreqSystem {
         std::vector<double> xd, zd;
         FreqCoef fc;
         freq_host_setup(m, nz, nx, N, nB, dz, dx, xd, zd, fc);
-        SSSI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        s.create();
         DevBuf<double> dm(n), dxd(n), dzd(n);
         SSSI_HIP(hipMemcpyAsync(dm.p, m, dm.bytes(), hipMemcpyHostToDevice, s));
         SSSI_HIP(hipMemcpyAsync(dxd.p, xd.data(), dxd.bytes(), hipMemcpyHostToDevice, s));
@@ -561,9 +561,6 @@ struct FreqSystem {
             SSSI_HIP(hipMemcpyAsync(val_out, dval.p, dval.bytes(), hipMemcpyDeviceToHost, s));
         }
         SSSI_HIP(hipStreamSynchronize(s));
-    }
-    ~FreqSystem() {
-        if (s) (void)hipStreamDestroy(s);
     }
 
     // dynamic LDS a workgroup may use on the current device

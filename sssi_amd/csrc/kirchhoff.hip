@@ -309,7 +309,7 @@ class Eikonal2D {
    public:
     int64_t nz = 0, nx = 0;
     int nsrc = 0;
-    hipStream_t stream = nullptr;
+    OwnedStream stream;  // before the buffers: destroyed after them
     DevBuf<double> slow, t[2];
     DevBuf<int64_t> src;
     int cur = 0;
@@ -331,7 +331,7 @@ class Eikonal2D {
                      sz[k], sx[k], (long long)nz, (long long)nx);
             cells[k] = (int64_t)(sz[k] - 1) + (int64_t)(sx[k] - 1) * nz;
         }
-        SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        stream.create();
         std::vector<double> hs(nn);
         for (int64_t q = 0; q < nn; ++q) hs[q] = 1 / v[q];  // the reference's slowness
         slow.alloc(nn);
@@ -380,9 +380,6 @@ class Eikonal2D {
         SSSI_HIP(hipEventElapsedTime(&pass_ms, ev[0], ev[1]));
         (void)hipEventDestroy(ev[0]);
         (void)hipEventDestroy(ev[1]);
-    }
-    ~Eikonal2D() {
-        if (stream) (void)hipStreamDestroy(stream);
     }
     const double *table() const { return t[cur].p; }
     void download(double *T) {

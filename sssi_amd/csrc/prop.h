@@ -47,7 +47,11 @@ struct SnapSink {
 
 class Prop {
    public:
-    virtual ~Prop() = default;
+    // the stream is destroyed here, so that a derived constructor that throws
+    // after creating it does not leak it
+    virtual ~Prop() {
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
+    }
     int dim = 2;
     int abc = SSSI_ABC_CPML;
     int64_t nz = 0, nx = 0, ny = 1;

@@ -184,8 +184,7 @@ class Prop2D final : public Prop {
     }
 
     ~Prop2D() override {
-        graph.clear();
-        if (stream && own_stream) (void)hipStreamDestroy(stream);
+        graph.clear();  // before ~Prop destroys the stream its replays ran on
     }
 
     void build_order() {

@@ -868,10 +868,6 @@ class Prop3D final : public Prop {
         }
     }
 
-    ~Prop3D() override {
-        if (stream && own_stream) (void)hipStreamDestroy(stream);
-    }
-
     // Autotuner: time every (ty, z chunk) candidate on this grid (a short
     // warm-up, then kTuneSteps steps timed with HIP events), keep the fastest,
     // cache it for the shape.  Runs before any source is set; the state is

@@ -85,13 +85,13 @@ static uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603u
 
 class Rtm2D {
    public:
+    OwnedStream stream;  // first member: destroyed after the device state
     int64_t nz = 0, nx = 0, nt = 0;
     int N = 4, nB = 0, stride = 1, fmt = kCkF32;
     double dz = 0, dx = 0, dt = 0, eps = 1e-3;
     std::vector<double> V, VS;  // physical models, column-major
     std::vector<std::unique_ptr<RtmShot>> shots;
     DevBuf<double> stack;       // nz x nx physical, column-major
-    hipStream_t stream = nullptr;
     int64_t launches = 0;
     // per-shot result cache (MAT-files in cache_dir, empty: off)
     std::string cache_dir;
@@ -103,12 +103,9 @@ class Rtm2D {
           dt(dt_), eps(eps_) {
         V.assign(V_, V_ + nz * nx);
         VS.assign(VS_, VS_ + nz * nx);
-        SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        stream.create();
         stack.alloc((size_t)nz * nx);
         SSSI_HIP(hipMemsetAsync(stack.p, 0, stack.bytes(), stream));
-    }
-    ~Rtm2D() {
-        if (stream) (void)hipStreamDestroy(stream);
     }
 
     int64_t nck() const { return (nt + stride - 1) / stride; }
