@@ -193,7 +193,10 @@ __device__ __forceinline__ void cpml3d_sweep_inner(const Cpml3dArgs &a, float *s
     constexpr int L = T::L, PX = T::PX, PY = T::PY, HPT = T::HPT;
     static_assert(RG == 2 || RG >= L + 2, "plane ring: 2 tiles, or L+2 and more");
     constexpr int HL = RG >= L + 2 ? L : 0;  // planes the halo loads run ahead
-    const int tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
+    // the wave's row of the tile as a scalar: the compiler cannot prove
+    // tid >> 6 uniform, and every row-dependent offset and test would be
+    // vector arithmetic
+    const int tid = threadIdx.x, lane = tid & 63, ty = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x0 = tx * kTX, y0 = tyi * TY;
     const int nz = a.nz, nx = a.nx, ny = a.ny;
     const int j = x0 + lane, k = y0 + ty;
@@ -357,7 +360,10 @@ __device__ __forceinline__ void cpml3d_sweep_strip(const Cpml3dArgs &a, float *s
                   QY = T::QY;
     static_assert(RG == 2 || RG >= L + 2, "plane ring: 2 tiles, or L+2 and more");
     constexpr int HL = RG >= L + 2 ? L : 0;  // planes the halo loads run ahead
-    const int tid = threadIdx.x, lane = tid & 63, ty = tid >> 6;
+    // the wave's row of the tile as a scalar: the compiler cannot prove
+    // tid >> 6 uniform, and every row-dependent offset and test would be
+    // vector arithmetic
+    const int tid = threadIdx.x, lane = tid & 63, ty = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x0 = tx * kTX, y0 = tyi * TY;
     const int nz = a.nz, nx = a.nx, ny = a.ny, nB = a.nB;
     const int j = x0 + lane, k = y0 + ty;
