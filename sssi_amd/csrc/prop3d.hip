@@ -393,37 +393,52 @@ __device__ __forceinline__ void cpml3d_sweep_strip(const Cpml3dArgs &a, float *s
     auto ysi = [&](int y) { return y < nB ? y : y - (ny - nb2); };  // y strip index
     auto in_xs = [&](int x) { return x >= 0 && x < nx && (x < nB || x >= nx - nB); };
     auto in_ys = [&](int y) { return y >= 0 && y < ny && (y < nB || y >= ny - nB); };
-    // byte offsets of this lane's strip values for plane i (kOOB: none)
+    // Byte offsets of this lane's strip values for plane i, as a lane part
+    // (kOOB for a lane without a value, fixed for the sweep) plus a
+    // wave-uniform part (kNone for a plane without values): kOOB + part,
+    // lane + kNone and kOOB + kNone all stay past the pool's end (pools stay
+    // below 2 GiB) without wrapping, so an offset costs one vector add.
+    constexpr uint32_t kNone = 0x7fffffffu;
+    const uint32_t l_zc = own ? ((uint32_t)k * nx + j) * 4u : kOOB;  // z cell arrays
+    const uint32_t l_j = own ? (uint32_t)j * 4u : kOOB;              // z profile, y own cells
+    const uint32_t l_jx = j < nx ? (uint32_t)j * 4u : kOOB;          // y half points
+    uint32_t l_xh[2];                                                  // x half points
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const int hx = x0 - N + lane + half * kTX;
+        l_xh[half] = (half == 0 || lane < L) && in_xs(hx) ? (uint32_t)xsi(hx) * 4u : kOOB;
+    }
+    const uint32_t l_xo = own && in_xs(j) ? (uint32_t)xsi(j) * 4u : kOOB;  // x own cells
     auto o_zh = [&](int i, uint32_t base, bool cell) -> uint32_t {  // half point i+N-1
         const int hz = i + N - 1;
-        if (!(own && hz >= zb0 && hz < nz)) return kOOB;
         const uint32_t r = (uint32_t)(hz - zb0);
-        return base + (cell ? (r * ny + k) * nx + j : r * nx + j) * 4u;
+        const uint32_t u = hz >= zb0 && hz < nz ? base + r * (cell ? ny * nx : nx) * 4u : kNone;
+        return (cell ? l_zc : l_j) + u;
     };
     auto o_zi = [&](int i, uint32_t base, bool cell) -> uint32_t {  // cell i
-        if (!(own && i < zend && i >= zb0)) return kOOB;
         const uint32_t r = (uint32_t)(i - zb0);
-        return base + (cell ? (r * ny + k) * nx + j : r * nx + j) * 4u;
+        const uint32_t u = i < zend && i >= zb0 ? base + r * (cell ? ny * nx : nx) * 4u : kNone;
+        return (cell ? l_zc : l_j) + u;
+    };
+    auto o_xu = [&](int i, uint32_t base, bool cell) -> uint32_t {  // uniform part, x arrays
+        return i < zend && k < ny
+                   ? base + (cell ? ((uint32_t)i * ny + k) * nb2 : (uint32_t)i * nb2) * 4u
+                   : kNone;
     };
     auto o_xh = [&](int i, int half, uint32_t base, bool cell) -> uint32_t {
-        const int hx = x0 - N + lane + half * kTX;
-        if (!(i < zend && k < ny && (half == 0 || lane < L) && in_xs(hx))) return kOOB;
-        const uint32_t xs = (uint32_t)xsi(hx);
-        return base + (cell ? ((uint32_t)i * ny + k) * nb2 + xs : (uint32_t)i * nb2 + xs) * 4u;
+        return l_xh[half] + o_xu(i, base, cell);
     };
     auto o_xo = [&](int i, uint32_t base, bool cell) -> uint32_t {
-        if (!(own && i < zend && in_xs(j))) return kOOB;
-        const uint32_t xs = (uint32_t)xsi(j);
-        return base + (cell ? ((uint32_t)i * ny + k) * nb2 + xs : (uint32_t)i * nb2 + xs) * 4u;
+        return l_xo + o_xu(i, base, cell);
     };
     auto o_yh = [&](int i, int u, uint32_t base) -> uint32_t {
         const int q = ty + u * TY, hy = y0 - N + q;
-        if (!(i < zend && j < nx && q < AYH && in_ys(hy))) return kOOB;
-        return base + (((uint32_t)i * nb2 + ysi(hy)) * nx + j) * 4u;
+        return l_jx + (i < zend && q < AYH && in_ys(hy)
+                           ? base + ((uint32_t)i * nb2 + ysi(hy)) * nx * 4u : kNone);
     };
     auto o_yo = [&](int i, uint32_t base) -> uint32_t {
-        if (!(own && i < zend && in_ys(k))) return kOOB;
-        return base + (((uint32_t)i * nb2 + ysi(k)) * nx + j) * 4u;
+        return l_j + (i < zend && in_ys(k) ? base + ((uint32_t)i * nb2 + ysi(k)) * nx * 4u
+                                           : kNone);
     };
 
     // this plane's strip values (loaded during the previous plane)
