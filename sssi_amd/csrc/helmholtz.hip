@@ -158,11 +158,18 @@ __global__ void __launch_bounds__(256) freq_panel(BandDesc B, int64_t j0, int64_
         else return B.at(i, j);
     };
     if constexpr (LDS) {
-        for (int c = 0; c < kNB; ++c)
-            for (int r = tid; r < Ri; r += 256) {
+        // a row's kNB entries are loaded together, then stored to LDS: one
+        // memory latency per row instead of one per entry
+        for (int r = tid; r < Ri; r += 256) {
+            cx v[kNB];
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) {
                 const int64_t i = j0 + r, j = j0 + c;
-                P[c * Ri + r] = inband(i, j) ? B.at(i, j) : cx{0.0, 0.0};
+                v[c] = inband(i, j) ? B.at(i, j) : cx{0.0, 0.0};
             }
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) P[c * Ri + r] = v[c];
+        }
     }
     if (tid == 0) s_ju = st->ju;
     __syncthreads();
@@ -210,17 +217,35 @@ __global__ void __launch_bounds__(256) freq_panel(BandDesc B, int64_t j0, int64_
             for (int t = 1 + tid; t <= km; t += 256) {
                 const cx l = cmul(A(j + t, j), r);
                 A(j + t, j) = l;
-                for (int64_t c = j + 1; c <= cend; ++c) A(j + t, c) = csub_mul(A(j + t, c), l, A(j, c));
+                // columns four at a time: their loads are independent and go
+                // out together (each entry's update is the same operation)
+                int64_t c = j + 1;
+                for (; c + 3 <= cend; c += 4) {
+                    cx a[4], u[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        a[q] = A(j + t, c + q);
+                        u[q] = A(j, c + q);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) A(j + t, c + q) = csub_mul(a[q], l, u[q]);
+                }
+                for (; c <= cend; ++c) A(j + t, c) = csub_mul(A(j + t, c), l, A(j, c));
             }
         }
         __syncthreads();
     }
     if constexpr (LDS) {
-        for (int c = 0; c < kNB; ++c)
-            for (int r = tid; r < Ri; r += 256) {
+        for (int r = tid; r < Ri; r += 256) {
+            cx v[kNB];
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) v[c] = P[c * Ri + r];
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) {
                 const int64_t i = j0 + r, j = j0 + c;
-                if (inband(i, j)) B.at(i, j) = P[c * Ri + r];
+                if (inband(i, j)) B.at(i, j) = v[c];
             }
+        }
     }
     if (tid == 0) st->ju = s_ju;
 }
@@ -266,16 +291,25 @@ __global__ void __launch_bounds__(256) freq_trail_lds(BandDesc B, int64_t j0,
     const int64_t n = B.n, kl = B.kl, ku = B.ku;
     const int64_t c = j0 + kNB + blockIdx.x;
     const int64_t jend = min(j0 + kNB, n);
-    if (c >= n || c > juv[jend - 1]) return;
     const int tid = threadIdx.x;
     auto inband = [&](int64_t i) { return i < n && i - c <= kl && c - i <= kl + ku; };
+    // Every load goes out before the first wait: the reaches and pivots, the
+    // column's rows and the panel's multipliers do not depend on one another,
+    // and each is in bounds whether or not the column turns out to be beyond
+    // the panel's reach (then the block leaves without using them).
+    const int64_t reach = juv[jend - 1];
     int64_t jv[kNB], pv[kNB];
 #pragma unroll
     for (int pp = 0; pp < kNB; ++pp) {
         jv[pp] = j0 + pp < jend ? juv[j0 + pp] : -1;
         pv[pp] = j0 + pp < jend ? piv[j0 + pp] : j0 + pp;
     }
-    for (int64_t r = tid; r < R; r += 256) Cl[r] = inband(j0 + r) ? B.at(j0 + r, c) : cx{0.0, 0.0};
+    cx col[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int64_t r = tid + 256 * q;
+        col[q] = r < R && c < n && inband(j0 + r) ? B.at(j0 + r, c) : cx{0.0, 0.0};
+    }
     // multiplier of local row r at step p: L(j0 + r, p) when p < j0 + r <= p + km(p)
     cx L[kNB][RPT];
 #pragma unroll
@@ -286,6 +320,12 @@ __global__ void __launch_bounds__(256) freq_trail_lds(BandDesc B, int64_t j0,
             L[pp][q] = (p < jend && i > p && i <= p + min(kl, n - 1 - p)) ? B.at(i, p)
                                                                          : cx{0.0, 0.0};
         }
+    if (c >= n || c > reach) return;  // beyond every step's reach: untouched
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int64_t r = tid + 256 * q;
+        if (r < R) Cl[r] = col[q];
+    }
     __syncthreads();
 #pragma unroll
     for (int pp = 0; pp < kNB; ++pp) {
