@@ -144,8 +144,6 @@ __global__ void __launch_bounds__(256) freq_panel(BandDesc B, int64_t j0, int64_
     extern __shared__ cx P[];
     __shared__ double s_best[4];
     __shared__ int s_arg[4];
-    __shared__ int s_jp;
-    __shared__ int64_t s_ju;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t n = B.n, kl = B.kl, ku = B.ku;
     const int64_t jend = min(j0 + kNB, n);
@@ -171,7 +169,8 @@ __global__ void __launch_bounds__(256) freq_panel(BandDesc B, int64_t j0, int64_
             for (int c = 0; c < kNB; ++c) P[c * Ri + r] = v[c];
         }
     }
-    if (tid == 0) s_ju = st->ju;
+    // furthest column any step so far reaches (kept by every thread alike)
+    int64_t ju = st->ju;
     __syncthreads();
     for (int64_t j = j0; j < jend; ++j) {
         const int km = (int)min(kl, n - 1 - j);
@@ -190,21 +189,21 @@ __global__ void __launch_bounds__(256) freq_panel(BandDesc B, int64_t j0, int64_
         }
         if (lane == 0) { s_best[wv] = best; s_arg[wv] = arg; }
         __syncthreads();
+        // every thread combines the four waves' candidates (the same first
+        // maximum), so no second barrier hands the pivot out
+        double b = s_best[0];
+        int a = s_arg[0];
+        for (int q = 1; q < 4; ++q)
+            if (s_best[q] > b || (s_best[q] == b && s_arg[q] < a)) { b = s_best[q]; a = s_arg[q]; }
+        const int64_t reach = min(j + ku + a, n - 1);
+        if (reach > ju) ju = reach;
         if (tid == 0) {
-            double b = s_best[0];
-            int a = s_arg[0];
-            for (int q = 1; q < 4; ++q)
-                if (s_best[q] > b || (s_best[q] == b && s_arg[q] < a)) { b = s_best[q]; a = s_arg[q]; }
             if (b == 0.0 && st->singular == 0) st->singular = (int)min<int64_t>(j + 1, INT32_MAX);
             piv[j] = j + a;
-            const int64_t reach = min(j + ku + a, n - 1);
-            if (reach > s_ju) s_ju = reach;
-            juv[j] = s_ju;
-            s_jp = a;
+            juv[j] = ju;
         }
-        __syncthreads();
-        const int jp = s_jp;
-        const int64_t cend = min(jend - 1, s_ju);  // panel columns within reach
+        const int jp = a;
+        const int64_t cend = min(jend - 1, ju);  // panel columns within reach
         if (jp != 0 && j + tid <= cend) {
             const int64_t c = j + tid;
             const cx t = A(j, c);
@@ -247,7 +246,7 @@ __global__ void __launch_bounds__(256) freq_panel(BandDesc B, int64_t j0, int64_
             }
         }
     }
-    if (tid == 0) st->ju = s_ju;
+    if (tid == 0) st->ju = ju;
 }
 
 // Replay the panel's swaps and updates on column c = j0 + kNB + blockIdx.x,
