@@ -291,17 +291,15 @@ __global__ void __launch_bounds__(256) freq_trail_lds(BandDesc B, int64_t j0,
     const int64_t c = j0 + kNB + blockIdx.x;
     const int64_t jend = min(j0 + kNB, n);
     const int tid = threadIdx.x;
+    __shared__ int64_t s_jv[kNB], s_pv[kNB];
     auto inband = [&](int64_t i) { return i < n && i - c <= kl && c - i <= kl + ku; };
-    // Every load goes out before the first wait: the reaches and pivots, the
-    // column's rows and the panel's multipliers do not depend on one another,
-    // and each is in bounds whether or not the column turns out to be beyond
-    // the panel's reach (then the block leaves without using them).
+    // The reaches, pivots and the column's rows go out together; the panel's
+    // multipliers are loaded one step ahead of their use (two register sets),
+    // which keeps the kernel small enough for 8 waves per SIMD.
     const int64_t reach = juv[jend - 1];
-    int64_t jv[kNB], pv[kNB];
-#pragma unroll
-    for (int pp = 0; pp < kNB; ++pp) {
-        jv[pp] = j0 + pp < jend ? juv[j0 + pp] : -1;
-        pv[pp] = j0 + pp < jend ? piv[j0 + pp] : j0 + pp;
+    if (tid < kNB) {
+        s_jv[tid] = j0 + tid < jend ? juv[j0 + tid] : -1;
+        s_pv[tid] = j0 + tid < jend ? piv[j0 + tid] : j0 + tid;
     }
     cx col[RPT];
 #pragma unroll
@@ -310,15 +308,16 @@ __global__ void __launch_bounds__(256) freq_trail_lds(BandDesc B, int64_t j0,
         col[q] = r < R && c < n && inband(j0 + r) ? B.at(j0 + r, c) : cx{0.0, 0.0};
     }
     // multiplier of local row r at step p: L(j0 + r, p) when p < j0 + r <= p + km(p)
-    cx L[kNB][RPT];
-#pragma unroll
-    for (int pp = 0; pp < kNB; ++pp)
+    auto lmul = [&](int pp, cx (&L)[RPT]) {
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int64_t p = j0 + pp, i = j0 + tid + 256 * q;
-            L[pp][q] = (p < jend && i > p && i <= p + min(kl, n - 1 - p)) ? B.at(i, p)
-                                                                         : cx{0.0, 0.0};
+            L[q] = (pp < kNB && p < jend && i > p && i <= p + min(kl, n - 1 - p)) ? B.at(i, p)
+                                                                                : cx{0.0, 0.0};
         }
+    };
+    cx La[RPT], Lb[RPT];
+    lmul(0, La);
     if (c >= n || c > reach) return;  // beyond every step's reach: untouched
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
@@ -326,26 +325,30 @@ __global__ void __launch_bounds__(256) freq_trail_lds(BandDesc B, int64_t j0,
         if (r < R) Cl[r] = col[q];
     }
     __syncthreads();
-#pragma unroll
-    for (int pp = 0; pp < kNB; ++pp) {
+    auto step = [&](int pp, const cx (&L)[RPT], cx (&Ln)[RPT]) {
+        lmul(pp + 1, Ln);
         const int64_t p = j0 + pp;
-        if (c <= jv[pp]) {  // block-uniform (jv = -1 past the last column)
-            const int64_t jp = pv[pp] - p;
-            if (jp != 0 && tid == 0) {
-                const cx t = Cl[pp];
-                Cl[pp] = Cl[pp + jp];
-                Cl[pp + jp] = t;
-            }
-            __syncthreads();
-            const cx u = Cl[pp];
-            const int64_t km = min(kl, n - 1 - p);
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                const int64_t r = tid + 256 * q, i = j0 + r;
-                if (i > p && i <= p + km) Cl[r] = csub_mul(Cl[r], L[pp][q], u);
-            }
-            __syncthreads();
+        if (c > s_jv[pp]) return;  // block-uniform (-1 past the last column)
+        const int64_t jp = s_pv[pp] - p;
+        if (jp != 0 && tid == 0) {
+            const cx t = Cl[pp];
+            Cl[pp] = Cl[pp + jp];
+            Cl[pp + jp] = t;
         }
+        __syncthreads();
+        const cx u = Cl[pp];
+        const int64_t km = min(kl, n - 1 - p);
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t r = tid + 256 * q, i = j0 + r;
+            if (i > p && i <= p + km) Cl[r] = csub_mul(Cl[r], L[q], u);
+        }
+        __syncthreads();
+    };
+#pragma unroll 1
+    for (int pp = 0; pp < kNB; pp += 2) {
+        step(pp, La, Lb);
+        step(pp + 1, Lb, La);
     }
     for (int64_t r = tid; r < R; r += 256)
         if (inband(j0 + r)) B.at(j0 + r, c) = Cl[r];
