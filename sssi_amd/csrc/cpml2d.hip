@@ -55,12 +55,11 @@ __device__ __forceinline__ float f16_to_32(uint16_t h, int fmt) {
 // Wave-uniform quantities of one time step of one job.
 //
 // 16-bit checkpoints carry one FP32 scale each.  BF16 has FP32's exponent
-// range, so its checkpoint c is written directly in its own step with the scale
-// taken from checkpoint c-1's |p| maximum (complete: written by an earlier
-// launch; 1 for c = 0) -- one 2-byte store per cell.  FP16 would overflow a
-// field that grows past twice the previous maximum, so its checkpoint c is
-// staged in FP32 and packed during the next step (or after the last one) with
-// its own maximum.
+// range, so its checkpoint c is written directly in its own step, unscaled
+// (scale 1: a scale would not change BF16's relative precision) -- one 2-byte
+// store per cell and no |p| maximum.  FP16 would overflow a field that grows
+// past twice the previous maximum, so its checkpoint c is staged in FP32 and
+// packed during the next step (or after the last one) with its own maximum.
 struct StepCtx {
     int t, par;
     uint32_t rowb, nbytes;
@@ -93,8 +92,8 @@ __device__ __forceinline__ StepCtx step_ctx(const Job2D &J, int step) {
     S.im_now = J.im_num && (S.t % J.im_stride == 0);
     S.ci_im = S.im_now ? S.t / J.im_stride : 0;
     S.scale = S.inv_scale = S.im_scale = 1.f;
-    if (S.pack_now || (S.direct_now && S.c > 0)) {
-        // pack: checkpoint c-1's own scale; direct: checkpoint c scaled by c-1's
+    if (S.pack_now) {
+        // checkpoint c-1's own scale
         S.scale = ck_scale_of(J, S.c - 1);
         S.inv_scale = 1.0f / S.scale;
     }
@@ -158,8 +157,7 @@ __device__ __forceinline__ void tile_update(const Job2D &J, const StepCtx &S,
                 if (J.ck_fmt == kCkF32) {
                     J.ck32[(size_t)S.c * ncell + cell] = pn;
                 } else if (S.direct_now) {
-                    J.ck16[(size_t)S.c * ncell + cell] = f32_to_16(pn * S.inv_scale, J.ck_fmt);
-                    wmax = fmaxf(wmax, fabsf(pn));
+                    J.ck16[(size_t)S.c * ncell + cell] = f32_to_16(pn, J.ck_fmt);
                 } else {
                     J.ck_stage[(size_t)(S.c & 1) * ncell + cell] = pn;
                     wmax = fmaxf(wmax, fabsf(pn));
@@ -190,10 +188,10 @@ __device__ __forceinline__ void pin(float (&v)[R]) {
     for (int r = 0; r < R; ++r) asm volatile("" : "+v"(v[r]));
 }
 
-// Per-wave |p| maximum of a 16-bit checkpoint into one of the kCkSlots slots.
+// Per-wave |p| maximum of a staged FP16 checkpoint into one of the kCkSlots slots.
 __device__ __forceinline__ void ck_max_commit(const Job2D &J, const StepCtx &S, float wmax,
                                               int slot, int lane) {
-    if (!S.stage_now && !S.direct_now) return;
+    if (!S.stage_now) return;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, off, 64));
     if (lane == 0)

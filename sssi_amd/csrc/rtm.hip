@@ -7,7 +7,7 @@
 //   stack      Stacked += M ./ (eps + s2)                     (:315)
 // Each shot runs on an extended sub-window of the model (extBoundary: nB
 // edge-replicated columns left/right and rows below).  Checkpoints are FP32 or
-// 16-bit (FP16/BF16) with a per-checkpoint scale; the per-shot accumulators are
+// 16-bit (FP16 with a per-checkpoint scale, BF16 unscaled); the per-shot accumulators are
 // FP32 and the stack FP64, accumulated in shot order (deterministic).
 //
 // Two execution modes over the same kernels: one shot per launch (each time
