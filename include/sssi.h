@@ -215,8 +215,8 @@ int sssi_prop_layout(sssi_prop *p, int *a, int *b);
  * right and below.  Per shot:
  *   dataTrue  = fwd(V, src)             (computed once and cached, :165)
  *   dataSmooth= fwd(VS, src), checkpoints of the source wavefield every
- *               img_stride steps (FP32, or FP16/BF16 with a per-checkpoint
- *               scale)
+ *               img_stride steps (FP32, BF16, or FP16 with a
+ *               per-checkpoint scale)
  *   rvs(V, dataTrue - dataSmooth)       (:252, :256) with the imaging
  *   condition M += f.*r, s2 += f.^2 at the checkpoint steps (:270-273)
  *   Stacked(window) += M ./ (eps + s2)  (:315), FP64, in shot order.
