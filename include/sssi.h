@@ -376,7 +376,7 @@ void sssi_rtm2d_destroy(sssi_rtm *r);
 
 /* ---------------------------------------------------------------------
  * Serving: a request loop with continuous batching.  Jobs are submitted at
- * any time (inputs are copied); a worker thread admits them when a slot is
+ * any time (inputs are copied, or borrowed where stated); a worker thread admits them when a slot is
  * free and their device footprint fits the HBM budget, advances every
  * resident job one time step per tick (the 2-D jobs of one nDiffOrder share
  * one launch), keeps wavefields and checkpoints resident until the job ends
@@ -407,6 +407,17 @@ int sssi_server_submit_rtm2d_data(sssi_server *s, const double *V, const double 
                                   double dz, double dx, double dt, int src_ix,
                                   const double *wavelet, const double *data_true, int img_stride,
                                   int ck_format, double eps, int64_t *job_id);
+/* The same job with borrowed inputs: V, VS and data_true are read when the
+ * job is set up on the device instead of copied at submission, so a burst of
+ * submissions returns at once.  The caller keeps them valid and unchanged
+ * until sssi_server_wait reports the job done.  No counterpart in the
+ * reference (serving only). */
+int sssi_server_submit_rtm2d_borrowed(sssi_server *s, const double *V, const double *VS,
+                                      int64_t nz, int64_t nx, int64_t nt, int nDiffOrder,
+                                      int nBoundary, double dz, double dx, double dt, int src_ix,
+                                      const double *wavelet, const double *data_true,
+                                      int img_stride, int ck_format, double eps,
+                                      int64_t *job_id);
 int sssi_server_submit_fwd3d(sssi_server *s, const double *v, int64_t nz, int64_t nx, int64_t ny,
                              int64_t nt, int nDiffOrder, int nBoundary, double dz, double dx,
                              double dy, double dt, int src_iz, int src_ix, int src_iy,

@@ -96,6 +96,9 @@ SIGNATURES = {
                                  _int, _D, _int, _int, _dbl, ctypes.POINTER(_i64)],
     "sssi_server_submit_rtm2d_data": [_vp, _D, _D, _i64, _i64, _i64, _int, _int, _dbl, _dbl, _dbl,
                                       _int, _D, _D, _int, _int, _dbl, ctypes.POINTER(_i64)],
+    "sssi_server_submit_rtm2d_borrowed": [_vp, _D, _D, _i64, _i64, _i64, _int, _int, _dbl, _dbl,
+                                          _dbl, _int, _D, _D, _int, _int, _dbl,
+                                          ctypes.POINTER(_i64)],
     "sssi_server_submit_fwd3d": [_vp, _D, _i64, _i64, _i64, _i64, _int, _int, _dbl, _dbl, _dbl,
                                  _dbl, _int, _int, _int, _D, ctypes.POINTER(_i64)],
     "sssi_server_wait": [_vp, _i64, _dbl, _I],

@@ -57,6 +57,9 @@ struct ServeJob {
     // inputs
     std::vector<double> v, vs, wavelet;
     std::vector<float> dtrue;  // RTM: cached dataTrue [t][j], or empty (computed)
+    // borrowed inputs (sssi_server_submit_rtm2d_borrowed): read when the job is
+    // set up instead of copied at submission; null once it is
+    const double *bv = nullptr, *bvs = nullptr, *bdtrue = nullptr;
     int64_t nz = 0, nx = 0, ny = 1, nt = 0;
     int N = 1, nB = 0, src_iz = 0, src_ix = 0, src_iy = 0, img_stride = 1, ck_fmt = kCkF32;
     double dz = 0, dx = 0, dy = 0, dt = 0, eps = 1e-3;
@@ -340,11 +343,13 @@ class Server {
             j.lane = 2 + (int)(j.id & 1);
             return;
         }
-        j.g.reset(new Cpml2dGrid(j.v.data(), j.nz, j.nx, j.N, j.nB, j.dz, j.dx, j.dt));
+        j.g.reset(new Cpml2dGrid(j.bv ? j.bv : j.v.data(), j.nz, j.nx, j.N, j.nB, j.dz, j.dx,
+                                 j.dt));
         j.src.set(*j.g, 1, &j.src_iz, &j.src_ix, j.wavelet.data(), j.nt);
         j.rec.alloc((size_t)j.nt * j.nx);
         if (j.kind == kRtm2d) {
-            j.gs.reset(new Cpml2dGrid(j.vs.data(), j.nz, j.nx, j.N, j.nB, j.dz, j.dx, j.dt));
+            j.gs.reset(new Cpml2dGrid(j.bvs ? j.bvs : j.vs.data(), j.nz, j.nx, j.N, j.nB, j.dz,
+                                      j.dx, j.dt));
             j.src_s.set(*j.gs, 1, &j.src_iz, &j.src_ix, j.wavelet.data(), j.nt);
             j.rec2.alloc((size_t)j.nt * j.nx);
             const size_t cells = (size_t)j.nz * j.nx;
@@ -360,6 +365,11 @@ class Server {
             j.num.alloc(cells);
             j.den.alloc(cells);
             j.img.alloc(cells);
+            if (j.bdtrue) {
+                // data(nx, nt) column-major is [t][j]: rounded once to FP32
+                j.dtrue.resize((size_t)j.nt * j.nx);
+                for (size_t q = 0; q < j.dtrue.size(); ++q) j.dtrue[q] = (float)j.bdtrue[q];
+            }
             if (!j.dtrue.empty()) {
                 SSSI_HIP(hipMemcpyAsync(j.rec.p, j.dtrue.data(), j.rec.bytes(),
                                         hipMemcpyHostToDevice, hs));
@@ -367,6 +377,7 @@ class Server {
                 std::vector<float>().swap(j.dtrue);
             }
         }
+        j.bv = j.bvs = j.bdtrue = nullptr;  // the caller's arrays are no longer read
         j.lane = group_lane(j.g->N);
         // an RTM shot with cached dataTrue starts at the fwd(VS) phase
         j.phase = j.kind == kRtm2d && j.have_true ? 1 : 0;
@@ -728,11 +739,15 @@ int sssi_server_submit_fwd2d(sssi_server *s, const double *v, int64_t nz, int64_
     });
 }
 
-int sssi_server_submit_rtm2d_data(sssi_server *s, const double *V, const double *VS, int64_t nz,
-                                  int64_t nx, int64_t nt, int nDiffOrder, int nBoundary,
-                                  double dz, double dx, double dt, int src_ix,
-                                  const double *wavelet, const double *data_true, int img_stride,
-                                  int ck_format, double eps, int64_t *job_id) {
+}  // extern "C"
+
+namespace {
+// RTM shot submission; with `borrow` the models and dataTrue are read when
+// the job is set up (the caller keeps them until the job is done).
+int submit_rtm(sssi_server *s, const double *V, const double *VS, int64_t nz, int64_t nx,
+               int64_t nt, int nDiffOrder, int nBoundary, double dz, double dx, double dt,
+               int src_ix, const double *wavelet, const double *data_true, int img_stride,
+               int ck_format, double eps, int64_t *job_id, bool borrow) {
     return serve_guard([&] {
         if (!V || !VS || !wavelet) fail(SSSI_EINVAL, "V, VS and wavelet must not be NULL");
         validate_common(2, nz, nx, 1, nt, nDiffOrder, nBoundary, dz, dx, 1.0, dt, true);
@@ -745,20 +760,46 @@ int sssi_server_submit_rtm2d_data(sssi_server *s, const double *V, const double 
         if (!(eps >= 0) || !std::isfinite(eps)) fail(SSSI_EINVAL, "eps must be finite, >= 0");
         auto j = std::make_unique<ServeJob>();
         j->kind = kRtm2d;
-        copy_in(j->v, V, (size_t)(nz * nx));
-        copy_in(j->vs, VS, (size_t)(nz * nx));
-        j->wavelet.assign(wavelet, wavelet + nt);
-        if (data_true) {
+        if (borrow) {
+            j->bv = V;
+            j->bvs = VS;
+            j->bdtrue = data_true;
+        } else {
+            copy_in(j->v, V, (size_t)(nz * nx));
+            copy_in(j->vs, VS, (size_t)(nz * nx));
             // data(nx, nt) column-major is [t][j]: rounded once to the device's FP32
-            copy_in(j->dtrue, data_true, (size_t)(nt * nx));
-            j->have_true = true;
+            if (data_true) copy_in(j->dtrue, data_true, (size_t)(nt * nx));
         }
+        j->have_true = data_true != nullptr;
+        j->wavelet.assign(wavelet, wavelet + nt);
         j->nz = nz; j->nx = nx; j->nt = nt; j->N = nDiffOrder; j->nB = nBoundary;
         j->dz = dz; j->dx = dx; j->dt = dt; j->src_iz = 0; j->src_ix = src_ix;
         j->img_stride = img_stride; j->ck_fmt = ck_format; j->eps = eps;
         const int64_t id = S(s).submit(std::move(j));
         if (job_id) *job_id = id;
     });
+}
+}  // namespace
+
+extern "C" {
+
+int sssi_server_submit_rtm2d_data(sssi_server *s, const double *V, const double *VS, int64_t nz,
+                                  int64_t nx, int64_t nt, int nDiffOrder, int nBoundary,
+                                  double dz, double dx, double dt, int src_ix,
+                                  const double *wavelet, const double *data_true, int img_stride,
+                                  int ck_format, double eps, int64_t *job_id) {
+    return submit_rtm(s, V, VS, nz, nx, nt, nDiffOrder, nBoundary, dz, dx, dt, src_ix, wavelet,
+                      data_true, img_stride, ck_format, eps, job_id, false);
+}
+
+int sssi_server_submit_rtm2d_borrowed(sssi_server *s, const double *V, const double *VS,
+                                      int64_t nz, int64_t nx, int64_t nt, int nDiffOrder,
+                                      int nBoundary, double dz, double dx, double dt, int src_ix,
+                                      const double *wavelet, const double *data_true,
+                                      int img_stride, int ck_format, double eps,
+                                      int64_t *job_id) {
+    return submit_rtm(s, V, VS, nz, nx, nt, nDiffOrder, nBoundary, dz, dx, dt, src_ix, wavelet,
+                      data_true, img_stride, ck_format, eps, job_id, true);
 }
 
 int sssi_server_submit_rtm2d(sssi_server *s, const double *V, const double *VS, int64_t nz,

@@ -19,11 +19,13 @@ class Server:
         _call("sssi_server_create", float(hbm_budget_bytes), int(max_active), ctypes.byref(h))
         self._h = h
         self._shapes = {}
+        self._borrowed = {}  # job id -> arrays the library reads until the job is done
 
     def close(self):
         if self._h:
             lib().sssi_server_destroy(self._h)
             self._h = None
+        self._borrowed.clear()
 
     def __del__(self):
         try:
@@ -48,9 +50,12 @@ class Server:
         return jid.value
 
     def submit_rtm2d(self, V, VS, nt, nDiffOrder, nBoundary, dz, dx, dt, src_ix, wavelet,
-                     img_stride=1, ck_format="fp32", eps=1e-3, data_true=None):
+                     img_stride=1, ck_format="fp32", eps=1e-3, data_true=None, borrow=True):
         """One RTM shot on the extended grid of V; with data_true (nx, nt), the
-        cached observed data, the job skips its fwd(V) propagation."""
+        cached observed data, the job skips its fwd(V) propagation.  With
+        borrow (the default) the library reads V, VS and data_true when it sets
+        the job up rather than copying them here; this object holds them until
+        the job is done, and the caller must not modify them before then."""
         V = _f64(V)
         VS = _f64(VS)
         w = np.ascontiguousarray(wavelet, dtype=np.float64)
@@ -60,11 +65,14 @@ class Server:
             if dtr.shape != (V.shape[1], int(nt)):
                 raise ValueError("data_true must be (nx, nt)")
         jid = ctypes.c_int64(0)
-        _call("sssi_server_submit_rtm2d_data", self._h, _dp(V), _dp(VS), V.shape[0], V.shape[1],
+        entry = "sssi_server_submit_rtm2d_borrowed" if borrow else "sssi_server_submit_rtm2d_data"
+        _call(entry, self._h, _dp(V), _dp(VS), V.shape[0], V.shape[1],
               int(nt), int(nDiffOrder), int(nBoundary), float(dz), float(dx), float(dt),
               int(src_ix), _dp(w), _dp(dtr), int(img_stride), CK_FORMATS[ck_format], float(eps),
               ctypes.byref(jid))
         self._shapes[jid.value] = V.shape
+        if borrow:
+            self._borrowed[jid.value] = (V, VS, dtr)
         return jid.value
 
     def submit_fwd3d(self, v, nt, nDiffOrder, nBoundary, dz, dx, dy, dt, src, wavelet):
@@ -81,15 +89,19 @@ class Server:
     def wait(self, job_id, timeout=-1.0):
         done = ctypes.c_int(0)
         _call("sssi_server_wait", self._h, int(job_id), float(timeout), ctypes.byref(done))
+        if done.value:
+            self._borrowed.pop(job_id, None)
         return bool(done.value)
 
     def drain(self):
         _call("sssi_server_drain", self._h)
+        self._borrowed.clear()
 
     def result(self, job_id):
         shape = self._shapes[job_id]
         out = np.zeros(shape, order="F")
         _call("sssi_server_result", self._h, int(job_id), _dp(out), int(out.size))
+        self._borrowed.pop(job_id, None)  # a job with a result is done
         return out
 
     def info(self, job_id):

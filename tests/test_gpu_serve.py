@@ -141,3 +141,33 @@ def test_served_rtm_with_cached_data_true_equals_computing_it():
         s.drain()
         ia, ib = s.result(a), s.result(b)
     assert np.any(ia != 0) and np.array_equal(ia, ib)
+
+
+def test_borrowed_rtm_inputs_equal_copied_ones():
+    """Borrowed submissions (the models and dataTrue read at job setup, held
+    by the Python object until the job is done) give the copied submissions'
+    images bit for bit, also when the arrays are temporaries that only the
+    server object keeps alive."""
+    kind, kw = make_jobs()[2]
+    nB = kw["nBoundary"]
+    V, VS = kw["V"], kw["VS"]
+    pz, px = V.shape[0] - nB, V.shape[1] - 2 * nB
+    r = sssi_amd.Rtm2D(V[:pz, nB:nB + px], VS[:pz, nB:nB + px], kw["nDiffOrder"], nB, kw["dz"],
+                       kw["dx"], kw["dt"], kw["nt"], img_stride=kw["img_stride"],
+                       ck_format=kw["ck_format"])
+    r.add_shot(0, px, kw["src_ix"] - nB, kw["wavelet"])
+    r.precompute_data()
+    dtrue = r.shot(0)["data_true"]
+    r.close()
+    rest = {k: v for k, v in kw.items() if k not in ("V", "VS")}
+    with sssi_amd.Server(hbm_budget_bytes=8e9, max_active=4) as s:
+        ids = [s.submit_rtm2d(**kw, borrow=False),
+               s.submit_rtm2d(**kw, data_true=dtrue, borrow=False),
+               s.submit_rtm2d(V=np.array(V, order="C"), VS=np.array(VS, order="C"), **rest),
+               s.submit_rtm2d(V=np.array(V, order="C"), VS=np.array(VS, order="C"), **rest,
+                              data_true=np.array(dtrue, order="C"))]
+        s.drain()
+        im = [s.result(i) for i in ids]
+    assert np.any(im[0] != 0)
+    for k in range(1, 4):
+        assert np.array_equal(im[0], im[k]), k
