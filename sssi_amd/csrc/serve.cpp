@@ -93,7 +93,7 @@ struct ServeJob {
 class Server {
    public:
     static constexpr int kLanes = 4;    // work streams: 0-1 2-D layouts, 2-3 3-D jobs
-    static constexpr int kHelpers = 4;  // threads building and releasing jobs
+    static constexpr int kHelpers = 8;  // threads building and releasing jobs
     static constexpr int kChunk2 = 16;  // time steps per tick of a 2-D layout
     static constexpr int kChunk3 = 4;   // time steps per tick of a 3-D job
     static constexpr int kAhead = 2;    // ticks in flight
