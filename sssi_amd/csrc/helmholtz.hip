@@ -26,6 +26,7 @@
 //                with the band window of the right-hand side kept in LDS.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -281,32 +282,36 @@ __global__ void __launch_bounds__(256) freq_trail(BandDesc B, int64_t j0,
 // thread's multipliers of the panel (rows tid + 256 q, q < RPT), the panel's
 // pivots and reaches loaded up front, so the 16 steps touch only LDS and
 // registers.
-template <int RPT>
+template <int RPT, int NC>
 __global__ void __launch_bounds__(256) freq_trail_lds(BandDesc B, int64_t j0,
                                                       const int64_t *__restrict__ piv,
                                                       const int64_t *__restrict__ juv, int64_t R) {
 #pragma clang fp contract(off)
-    __shared__ cx Cl[RPT * 256];
+    __shared__ cx Cl[NC][RPT * 256];
+    __shared__ int64_t s_jv[kNB], s_pv[kNB];
     const int64_t n = B.n, kl = B.kl, ku = B.ku;
-    const int64_t c = j0 + kNB + blockIdx.x;
+    const int64_t c0 = j0 + kNB + (int64_t)blockIdx.x * NC;  // columns c0 .. c0+NC-1
     const int64_t jend = min(j0 + kNB, n);
     const int tid = threadIdx.x;
-    __shared__ int64_t s_jv[kNB], s_pv[kNB];
-    auto inband = [&](int64_t i) { return i < n && i - c <= kl && c - i <= kl + ku; };
-    // The reaches, pivots and the column's rows go out together; the panel's
-    // multipliers are loaded one step ahead of their use (two register sets),
-    // which keeps the kernel small enough for 8 waves per SIMD.
+    auto inband = [&](int64_t i, int64_t c) {
+        return c < n && i < n && i - c <= kl && c - i <= kl + ku;
+    };
+    // The reaches, pivots and the columns' rows go out together; the panel's
+    // multipliers are loaded one step ahead of their use (two register sets)
+    // and serve all NC columns of the workgroup.
     const int64_t reach = juv[jend - 1];
     if (tid < kNB) {
         s_jv[tid] = j0 + tid < jend ? juv[j0 + tid] : -1;
         s_pv[tid] = j0 + tid < jend ? piv[j0 + tid] : j0 + tid;
     }
-    cx col[RPT];
+    cx col[NC][RPT];
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int64_t r = tid + 256 * q;
-        col[q] = r < R && c < n && inband(j0 + r) ? B.at(j0 + r, c) : cx{0.0, 0.0};
-    }
+    for (int k = 0; k < NC; ++k)
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t r = tid + 256 * q;
+            col[k][q] = r < R && inband(j0 + r, c0 + k) ? B.at(j0 + r, c0 + k) : cx{0.0, 0.0};
+        }
     // multiplier of local row r at step p: L(j0 + r, p) when p < j0 + r <= p + km(p)
     auto lmul = [&](int pp, cx (&L)[RPT]) {
 #pragma unroll
@@ -318,30 +323,38 @@ __global__ void __launch_bounds__(256) freq_trail_lds(BandDesc B, int64_t j0,
     };
     cx La[RPT], Lb[RPT];
     lmul(0, La);
-    if (c >= n || c > reach) return;  // beyond every step's reach: untouched
+    if (c0 >= n || c0 > reach) return;  // every column beyond every step's reach: untouched
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int64_t r = tid + 256 * q;
-        if (r < R) Cl[r] = col[q];
-    }
+    for (int k = 0; k < NC; ++k)
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t r = tid + 256 * q;
+            if (r < R) Cl[k][r] = col[k][q];
+        }
     __syncthreads();
     auto step = [&](int pp, const cx (&L)[RPT], cx (&Ln)[RPT]) {
         lmul(pp + 1, Ln);
-        const int64_t p = j0 + pp;
-        if (c > s_jv[pp]) return;  // block-uniform (-1 past the last column)
+        const int64_t p = j0 + pp, jv = s_jv[pp];
+        if (c0 > jv) return;  // block-uniform: no column of the block within reach
         const int64_t jp = s_pv[pp] - p;
-        if (jp != 0 && tid == 0) {
-            const cx t = Cl[pp];
-            Cl[pp] = Cl[pp + jp];
-            Cl[pp + jp] = t;
+        // column k within reach (block-uniform; the columns within reach are a prefix)
+        if (jp != 0 && tid < NC && c0 + tid <= jv && c0 + tid < n) {
+            cx *C = Cl[tid];
+            const cx t = C[pp];
+            C[pp] = C[pp + jp];
+            C[pp + jp] = t;
         }
         __syncthreads();
-        const cx u = Cl[pp];
         const int64_t km = min(kl, n - 1 - p);
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int64_t r = tid + 256 * q, i = j0 + r;
-            if (i > p && i <= p + km) Cl[r] = csub_mul(Cl[r], L[q], u);
+        for (int k = 0; k < NC; ++k) {
+            if (c0 + k > jv) break;
+            const cx u = Cl[k][pp];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int64_t r = tid + 256 * q, i = j0 + r;
+                if (i > p && i <= p + km) Cl[k][r] = csub_mul(Cl[k][r], L[q], u);
+            }
         }
         __syncthreads();
     };
@@ -350,8 +363,10 @@ __global__ void __launch_bounds__(256) freq_trail_lds(BandDesc B, int64_t j0,
         step(pp, La, Lb);
         step(pp + 1, Lb, La);
     }
-    for (int64_t r = tid; r < R; r += 256)
-        if (inband(j0 + r)) B.at(j0 + r, c) = Cl[r];
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+        for (int64_t r = tid; r < R; r += 256)
+            if (inband(j0 + r, c0 + k)) B.at(j0 + r, c0 + k) = Cl[k][r];
 }
 
 // One right-hand side per workgroup of 1024: b = -s, forward sweep with
@@ -628,6 +643,16 @@ struct FreqSystem {
         const size_t plds = (size_t)kNB * R * sizeof(cx);
         const bool panel_lds = plds + 1024 <= lds_limit();
         const int rpt = R <= 256 ? 1 : R <= 512 ? 2 : R <= 1024 ? 4 : 0;
+        // columns per workgroup of the replay: two share one load of the
+        // panel's multipliers (every workgroup reads the same panel, so fewer
+        // workgroups contend less for its cache lines).  SSSI_FREQ_NC = 1 or 4
+        // for the measured alternatives: 1050 replays take 67.2 ms at 2 columns,
+        // 77.3 ms at 1 and 78.5 ms at 4 on the 120 x 140, nDiffOrder 2 workload.
+        int trail_nc = 2;
+        if (const char *e = std::getenv("SSSI_FREQ_NC")) {
+            const int v = std::atoi(e);
+            trail_nc = v == 4 ? 4 : v == 1 ? 1 : 2;
+        }
         for (int64_t j0 = 0; j0 < n; j0 += kNB) {
             if (panel_lds)
                 hipLaunchKernelGGL(freq_panel<true>, dim3(1), dim3(256), plds, s, B, j0, piv.p,
@@ -638,10 +663,17 @@ struct FreqSystem {
             const int64_t c0 = j0 + kNB, c1 = std::min(n - 1, j0 + kNB - 1 + kl + ku);
             if (c1 < c0) continue;
             const dim3 g((unsigned)(c1 - c0 + 1));
-            switch (rpt) {
-                case 1: hipLaunchKernelGGL(freq_trail_lds<1>, g, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
-                case 2: hipLaunchKernelGGL(freq_trail_lds<2>, g, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
-                case 4: hipLaunchKernelGGL(freq_trail_lds<4>, g, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
+            const dim3 g2((unsigned)((c1 - c0 + 2) / 2)), g4((unsigned)((c1 - c0 + 4) / 4));
+            switch (rpt * (trail_nc == 4 ? 100 : trail_nc == 2 ? 10 : 1)) {
+                case 1: hipLaunchKernelGGL((freq_trail_lds<1, 1>), g, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
+                case 2: hipLaunchKernelGGL((freq_trail_lds<2, 1>), g, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
+                case 4: hipLaunchKernelGGL((freq_trail_lds<4, 1>), g, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
+                case 10: hipLaunchKernelGGL((freq_trail_lds<1, 2>), g2, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
+                case 20: hipLaunchKernelGGL((freq_trail_lds<2, 2>), g2, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
+                case 40: hipLaunchKernelGGL((freq_trail_lds<4, 2>), g2, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
+                case 100: hipLaunchKernelGGL((freq_trail_lds<1, 4>), g4, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
+                case 200: hipLaunchKernelGGL((freq_trail_lds<2, 4>), g4, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
+                case 400: hipLaunchKernelGGL((freq_trail_lds<4, 4>), g4, dim3(256), 0, s, B, j0, piv.p, juv.p, R); break;
                 default: hipLaunchKernelGGL(freq_trail, g, dim3(256), 0, s, B, j0, piv.p, juv.p);
             }
         }

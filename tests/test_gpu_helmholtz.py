@@ -56,6 +56,25 @@ def test_matrix_and_solution_match_oracle_bit_for_bit(nz, nx, N, nB, f):
         assert np.linalg.norm(r) <= 1e-11 * np.linalg.norm(S[:, s])
 
 
+@pytest.mark.parametrize("nc", ["1", "4"])
+@pytest.mark.parametrize("nz, nx, N, nB, f", [CASES[0], CASES[2], CASES[5], CASES[6]])
+def test_replay_columns_per_workgroup_match_oracle(nz, nx, N, nB, f, nc, monkeypatch):
+    """The column replay with one or four columns per workgroup (SSSI_FREQ_NC;
+    two, the default, is what the other tests run) gives the oracle's solution
+    bit for bit, including column counts that are not a multiple of the
+    workgroup's and a band that needs two rows per thread."""
+    monkeypatch.setenv("SSSI_FREQ_NC", nc)
+    m = _model(nz, nx, nz + nx)
+    w = 2 * np.pi * f
+    n = nz * nx
+    S = np.zeros((n, 2), dtype=np.complex128)
+    S[(nx // 2) * nz, 0] = 1.0
+    S[:, 1] = np.random.default_rng(nx).standard_normal(n)
+    print(f"helmholtz nc={nc} {nz}x{nx} N={N} nB={nB}", flush=True)
+    U = helmholtz.solve(m, S, w, N, nB, 10.0, 12.5)
+    assert np.array_equal(U, oracle.freq_cpml_2d_solve(m, S, w, N, nB, 10.0, 12.5))
+
+
 def test_drop_in_shapes_and_outputs():
     nz, nx = 20, 24
     m = _model(nz, nx, 5)
