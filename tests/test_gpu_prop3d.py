@@ -148,3 +148,33 @@ def test_plane_ring_equals_two_tile_buffer(N, ty, zc, pd, kernel_env):
     d_ref = oracle.fwd_cpml_3d(v, nt, N, nB, dz, dx, dy, dt, src_iz=iz, src_ix=ix, src_iy=iy,
                                wavelet=ww)[0]
     assert rel_l2(out["1"][0], d_ref) < TOL, rel_l2(out["1"][0], d_ref)
+
+
+@pytest.mark.parametrize("band", ["0", "1"])
+def test_strip_block_orders_equal_default(band, kernel_env):
+    """SSSI_STRIPBAND3 = 0 (natural order) and 1 (one banded list) only change
+    where the strip blocks run: the results equal the default order's (banded
+    per kind of strip) bit for bit."""
+    os.environ["SSSI_TY3"] = "8"
+    os.environ["SSSI_ZC3"] = "12"
+    nz, nx, ny, nB, N, nt = 40, 150, 60, 6, 4, 70
+    v = synth.cake_3d_dipping(nz, nx, ny, 3, 1500.0, 3000.0, seed=31)
+    dx = dy = dz = 10.0
+    dt = 0.4 * dx / 3000.0 / np.sqrt(3) / sum(abs(c) for c in synth.dCoef(N))
+    w = synth.ricker(25, nt, dt)
+    iz, ix, iy = [0, 30], [75, 10], [30, 50]
+    ww = np.stack([w, 0.4 * w], axis=1)
+    saved = os.environ.get("SSSI_STRIPBAND3")
+    try:
+        os.environ.pop("SSSI_STRIPBAND3", None)
+        base = sssi_amd.fwd_cpml_3d_points(v, nt, N, nB, dz, dx, dy, dt, iz, ix, iy, ww,
+                                           want_snapshot=True, snap_stride=9)
+        os.environ["SSSI_STRIPBAND3"] = band
+        alt = sssi_amd.fwd_cpml_3d_points(v, nt, N, nB, dz, dx, dy, dt, iz, ix, iy, ww,
+                                          want_snapshot=True, snap_stride=9)
+    finally:
+        if saved is None:
+            os.environ.pop("SSSI_STRIPBAND3", None)
+        else:
+            os.environ["SSSI_STRIPBAND3"] = saved
+    assert np.array_equal(base[0], alt[0]) and np.array_equal(base[1], alt[1])
