@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 
 def workload():
     from sssi_amd import workloads
-    return workloads.config2()
+    return workloads.cpml2d_1040x3080()
 
 
 def traffic_from_profiles(name="cpml2d_step_bench_traffic.json"):
@@ -99,7 +99,7 @@ def cpu_baseline(wl, steps=None, cores=None):
 
 def workload_3d():
     from sssi_amd import workloads
-    return workloads.config4()
+    return workloads.cpml3d_420x440x440()
 
 
 def bench_3d(args):
@@ -191,11 +191,11 @@ def bench_sponge(args):
 
 def workload_rtm(nshots=64, seed=5):
     from sssi_amd import workloads
-    return workloads.config3(nshots, seed)
+    return workloads.rtm_600x6000(nshots, seed)
 
 
 def bench_rtm(args):
-    """Config-3 RTM: 64 shots of uneven aperture on the 600 x 6000 model.
+    """RTM: 64 shots of uneven aperture on the 600 x 6000 model.
 
     Modes: plain (one shot per launch, FP32 checkpoints); batched (one launch per
     time step covers every shot) with FP32 and with 16-bit scaled checkpoints,
@@ -307,7 +307,7 @@ def bench_rtm(args):
 
 def serve_jobs(njobs, seed=7):
     from sssi_amd import workloads
-    return workloads.config5(njobs, seed)
+    return workloads.serving_mix(njobs, seed)
 
 
 def bench_serve(args):

@@ -1,21 +1,21 @@
-"""Seeded synthetic workloads of the five benchmark configurations.
+"""Seeded synthetic workloads: the benchmark cases and the Kirchhoff and Helmholtz ones.
 
 Each builder returns plain numpy inputs (models indexed [iz, ix(, iy)], Ricker
 wavelets, source positions) so that the same inputs drive the GPU path, the
 tests and the benchmark.
 
-  config1()      2-D CPML forward, 100 x 200 + nB 20 (120 x 240), N 2, nt 1000,
-                 dt 1 ms, 20 Hz Ricker at (0, 100 + nB), every-10th snapshots
-  config2()      2-D CPML forward, 1000 x 3000 + nB 40 (1040 x 3080), N 5,
-                 nt 4000, dt 0.5 ms, 25 Hz Ricker at the surface centre
-  config3(n)     2-D RTM, n shots (64) of width U[400, 1200] on a 600 x 6000
-                 layered model and its 10-cell Gaussian blur, N 4, nB 30,
-                 nt 3000, dt 1 ms, 15 Hz, imaging every 5 steps
-  config4()      3-D CPML forward, 400^3 + nB 20 (420 x 440 x 440), N 4,
-                 nt 2000, dt 1 ms, 15 Hz Ricker at the surface centre
-  config5(n)     serving mix: 70 % 2-D forward / RTM-shot jobs (200..2000 x
-                 200..4000, N 2..5, nt 500..3000), 30 % 3-D forward (128^3 ..
-                 256^3, nt 500..1500); layered 1500-4500 m/s; 10-30 Hz Ricker
+  cpml2d_120x240()       2-D CPML forward, 100 x 200 + nB 20 (120 x 240), N 2, nt 1000,
+                         dt 1 ms, 20 Hz Ricker at (0, 100 + nB), every-10th snapshots
+  cpml2d_1040x3080()     2-D CPML forward, 1000 x 3000 + nB 40 (1040 x 3080), N 5,
+                         nt 4000, dt 0.5 ms, 25 Hz Ricker at the surface centre
+  rtm_600x6000(n)        2-D RTM, n shots (64) of width U[400, 1200] on a 600 x 6000
+                         layered model and its 10-cell Gaussian blur, N 4, nB 30,
+                         nt 3000, dt 1 ms, 15 Hz, imaging every 5 steps
+  cpml3d_420x440x440()   3-D CPML forward, 400^3 + nB 20 (420 x 440 x 440), N 4,
+                         nt 2000, dt 1 ms, 15 Hz Ricker at the surface centre
+  serving_mix(n)         serving mix: 70 % 2-D forward / RTM-shot jobs (200..2000 x
+                         200..4000, N 2..5, nt 500..3000), 30 % 3-D forward (128^3 ..
+                         256^3, nt 500..1500); layered 1500-4500 m/s; 10-30 Hz Ricker
   kirchhoff(n)   Kirchhoff migration in the geometry of the reference's
                  Kirchhoff app (shots and receivers at every surface column,
                  tables for every position): 300 x 600, d 10 m, velocity
@@ -32,7 +32,7 @@ import numpy as np
 from . import synth
 
 
-def config1():
+def cpml2d_120x240():
     nz0, nx0, nB, N = 100, 200, 20, 2
     d, dt, nt = 10.0, 1e-3, 1000
     v = synth.layered_2d(nz0, nx0, 5, 1500.0, 4500.0, seed=1)
@@ -42,7 +42,7 @@ def config1():
                 src_ix=[nx0 // 2 + nB], snap_stride=10)
 
 
-def config2():
+def cpml2d_1040x3080():
     nz0, nx0, nB, N = 1000, 3000, 40, 5
     dz = dx = 5.0
     dt, nt = 5e-4, 4000
@@ -55,7 +55,7 @@ def config2():
                 src_ix=[nx0 // 2 + nB])
 
 
-def config3(nshots=64, seed=5):
+def rtm_600x6000(nshots=64, seed=5):
     nz, nx, nB, N = 600, 6000, 30, 4
     d, dt, nt = 10.0, 1e-3, 3000
     V = synth.layered_2d(nz, nx, 12, 1500.0, 4500.0, seed=seed)
@@ -71,7 +71,7 @@ def config3(nshots=64, seed=5):
                 img_stride=5)
 
 
-def config4():
+def cpml3d_420x440x440():
     n0, nB, N = 400, 20, 4
     d, dt, nt = 10.0, 1e-3, 2000
     v = synth.cake_3d_dipping(n0, n0, n0, 8, 1500.0, 4500.0, seed=4)
@@ -83,7 +83,7 @@ def config4():
     return dict(V=V, nB=nB, N=N, d=d, dt=dt, nt=nt, wavelet=w, src=(0, c, c))
 
 
-def config5(njobs=60, seed=7):
+def serving_mix(njobs=60, seed=7):
     """Job list [(kind, kwargs)] for sssi_amd.Server.submit_<kind>.  RTM-shot
     jobs carry an extBoundary-extended model (nB = 20) so that each one is also
     a one-shot Rtm2D migration of the physical model over its whole width."""
@@ -123,7 +123,7 @@ def config5(njobs=60, seed=7):
 
 
 def job_cells(kind, kw):
-    """Cell updates of a config5 job (RTM shots propagate three times)."""
+    """Cell updates of a serving-mix job (RTM shots propagate three times)."""
     if kind == "fwd3d":
         return int(np.prod(kw["v"].shape)) * kw["nt"]
     v = kw["v"] if kind == "fwd2d" else kw["V"]

@@ -1,16 +1,16 @@
-"""The benchmark configurations at their real sizes, on the GPU.
+"""The benchmark workloads at their real sizes, on the GPU.
 
-Config 1: 2-D CPML forward, 120 x 240 extended grid, nDiffOrder 2, nt 1000,
+2-D 120 x 240: CPML forward, 120 x 240 extended grid, nDiffOrder 2, nt 1000,
   every-10th snapshots, against the FP64 oracle (data rel-L2 <= 1e-5,
   snapshot stack rel-L2 <= 2e-5), through the point-source entry and through
   the MEX gateway with the reference's dense (nz, nx, nt) source, bit for bit.
-Config 3: the 64-shot RTM batch on the 600 x 6000 model with BF16
+RTM 600 x 6000: the 64-shot RTM batch on the 600 x 6000 model with BF16
   checkpoints: every per-shot image and the stack equal, bit for bit, the
   one-shot-per-launch migration of the same shots; two shots and their stack
   against the FP64 oracle (rel-L2 <= 1e-3).
-Config 4: 3-D CPML forward on the 420 x 440 x 440 grid for the full 2000
+3-D 420 x 440 x 440: CPML forward on the 420 x 440 x 440 grid for the full 2000
   steps: exact odd symmetry and time shift (no oracle at this size).
-Config 5: the seeded 60-job serving mix at its real sizes through the server:
+Serving mix: the seeded 60-job serving mix at its real sizes through the server:
   each served result equals its direct propagation bit for bit (2-D forward:
   the drop-in entry; RTM shot: a one-shot migration; 3-D forward: the drop-in
   entry), and the smallest job of each kind matches the FP64 oracle.
@@ -30,19 +30,19 @@ def rel_l2(a, b):
 
 
 @pytest.mark.timeout(300)
-def test_config1_matches_oracle_and_gateway():
+def test_cpml2d_120x240_matches_oracle_and_gateway():
     from tests import mexmock
-    wl = workloads.config1()
+    wl = workloads.cpml2d_120x240()
     V, nt, k = wl["V"], wl["nt"], wl["snap_stride"]
     args = (V, nt, wl["N"], wl["nB"], wl["dz"], wl["dx"], wl["dt"])
-    print(f"config 1: {V.shape} N={wl['N']} nt={nt} snap every {k}", flush=True)
+    print(f"2-D 120x240: {V.shape} N={wl['N']} nt={nt} snap every {k}", flush=True)
     d, s = sssi_amd.fwd_cpml_2d_points(*args, wl["src_iz"], wl["src_ix"], wl["wavelet"],
                                        want_snapshot=True, snap_stride=k)
     dr, sr = oracle.fwd_cpml_2d(*args, src_iz=wl["src_iz"], src_ix=wl["src_ix"],
                                 wavelet=wl["wavelet"], want_snapshot=True, snap_stride=k)
     e_d, e_s = rel_l2(d, dr), rel_l2(s, sr)
     e_slice = max(rel_l2(s[..., q], sr[..., q]) for q in range(1, s.shape[2]))
-    print(f"config 1: data rel-L2 {e_d:.3e}, snapshot stack {e_s:.3e}, worst slice "
+    print(f"2-D 120x240: data rel-L2 {e_d:.3e}, snapshot stack {e_s:.3e}, worst slice "
           f"{e_slice:.3e}", flush=True)
     assert e_d < 1e-5
     assert e_s < 2e-5
@@ -62,15 +62,15 @@ def test_config1_matches_oracle_and_gateway():
 
 
 @pytest.mark.timeout(600)
-def test_config3_full_batch_equals_one_shot_per_launch_and_oracle():
-    wl = workloads.config3()
+def test_rtm_600x6000_full_batch_equals_one_shot_per_launch_and_oracle():
+    wl = workloads.rtm_600x6000()
     shots = wl["shots"]
     kw = (wl["V"], wl["VS"], wl["N"], wl["nB"], wl["d"], wl["d"], wl["dt"], wl["nt"])
-    print(f"config 3: {len(shots)} shots, bf16 checkpoints", flush=True)
+    print(f"RTM 600x6000: {len(shots)} shots, bf16 checkpoints", flush=True)
     r = sssi_amd.Rtm2D(*kw, img_stride=wl["img_stride"], ck_format="bf16")
     for xb, xc, sx in shots:
         r.add_shot(xb, xc, sx, wl["wavelet"])
-    print(f"config 3: device {r.device_bytes() / 1e9:.1f} GB", flush=True)
+    print(f"RTM 600x6000: device {r.device_bytes() / 1e9:.1f} GB", flush=True)
     r.precompute_data(batched=True)
     r.migrate(batched=True)
     stack_b = r.stack()
@@ -95,7 +95,7 @@ def test_config3_full_batch_equals_one_shot_per_launch_and_oracle():
                                     nB, wl["d"], wl["d"], wl["dt"], img_stride=wl["img_stride"],
                                     eps=1e-3)
         e = rel_l2(img_b[q], o["image"])
-        print(f"config 3: shot {q} ({Vw.shape[0]}x{Vw.shape[1]}) image rel-L2 {e:.3e}",
+        print(f"RTM 600x6000: shot {q} ({Vw.shape[0]}x{Vw.shape[1]}) image rel-L2 {e:.3e}",
               flush=True)
         assert e < 1e-3
         stack_o[:, xb:xb + xc] += o["image"][:wl["V"].shape[0], nB:nB + xc]
@@ -103,19 +103,19 @@ def test_config3_full_batch_equals_one_shot_per_launch_and_oracle():
     for n, q in enumerate(pick):  # a shot's image does not depend on its batch
         assert np.array_equal(r2.shot(n)["image"], img_b[q])
     e = rel_l2(r2.stack(), stack_o)
-    print(f"config 3: 2-shot stack rel-L2 {e:.3e}", flush=True)
+    print(f"RTM 600x6000: 2-shot stack rel-L2 {e:.3e}", flush=True)
     assert e < 1e-3
     r2.close()
 
 
 @pytest.mark.timeout(600)
-def test_config4_full_length_odd_symmetry_and_time_shift():
-    wl = workloads.config4()
+def test_cpml3d_420x440x440_full_length_odd_symmetry_and_time_shift():
+    wl = workloads.cpml3d_420x440x440()
     nt = wl["nt"]
     args = (wl["V"], nt, wl["N"], wl["nB"], wl["d"], wl["d"], wl["d"], wl["dt"])
     iz, ix, iy = wl["src"]
     w = wl["wavelet"]
-    print(f"config 4: {wl['V'].shape} nt={nt}", flush=True)
+    print(f"3-D 420x440x440: {wl['V'].shape} nt={nt}", flush=True)
     d, _ = sssi_amd.fwd_cpml_3d_points(*args, [iz], [ix], [iy], w)
     assert np.any(d != 0) and np.all(np.isfinite(d))
     dn, _ = sssi_amd.fwd_cpml_3d_points(*args, [iz], [ix], [iy], -w)
@@ -172,24 +172,24 @@ def _oracle(kind, kw):
 
 
 @pytest.mark.timeout(900)
-def test_config5_served_mix_equals_direct_and_oracle():
-    jobs = workloads.config5(60)
+def test_serving_mix_served_mix_equals_direct_and_oracle():
+    jobs = workloads.serving_mix(60)
     kinds = [k for k, _ in jobs]
-    print("config 5: " + ", ".join(f"{k} x{kinds.count(k)}" for k in sorted(set(kinds))),
+    print("serving mix: " + ", ".join(f"{k} x{kinds.count(k)}" for k in sorted(set(kinds))),
           flush=True)
     with sssi_amd.Server(hbm_budget_bytes=200e9, max_active=64) as s:
         ids = [getattr(s, "submit_" + kind)(**kw) for kind, kw in jobs]
         s.drain()
         served = [s.result(i) for i in ids]
-    print("config 5: served", flush=True)
+    print("serving mix: served", flush=True)
     for q, ((kind, kw), res) in enumerate(zip(jobs, served)):
         ref = _direct(kind, kw)
         assert ref.shape == res.shape, (q, kind)
         assert np.array_equal(ref, res), (q, kind, rel_l2(res, ref))
-    print("config 5: every served result equals its direct run", flush=True)
+    print("serving mix: every served result equals its direct run", flush=True)
     for kind, tol in (("fwd2d", 1e-5), ("rtm2d", 1e-3), ("fwd3d", 1e-5)):
         cand = [q for q in range(len(jobs)) if jobs[q][0] == kind]
         q = min(cand, key=lambda q: workloads.job_cells(*jobs[q]))
         e = rel_l2(served[q], _oracle(*jobs[q]))
-        print(f"config 5: job {q} ({kind}) vs FP64 oracle rel-L2 {e:.3e}", flush=True)
+        print(f"serving mix: job {q} ({kind}) vs FP64 oracle rel-L2 {e:.3e}", flush=True)
         assert e < tol
