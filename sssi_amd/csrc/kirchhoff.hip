@@ -258,6 +258,64 @@ __global__ void __launch_bounds__(256) kirch_stack(const int *__restrict__ kidx,
         if (oob) atomicOr(bad, 1);
     }
 }
+// Shot groups in parallel: workgroup row blockIdx.y migrates shots
+// S*y .. S*y+S-1 (the loop of kirch_stack for one group) and writes their
+// images; kirch_sum then stacks the images in shot order, the same additions
+// in the same order as kirch_stack's running sum.  Four times the workgroups
+// of kirch_stack at 16 shots, for the latency of the record gathers.
+template <int S>
+__global__ void __launch_bounds__(256) kirch_group(const int *__restrict__ kidx, int64_t nn,
+                                                   const int *__restrict__ ts, int nshot,
+                                                   const int *__restrict__ tr,
+                                                   const int *__restrict__ xr, int nrec,
+                                                   const double *__restrict__ shots, int64_t nt,
+                                                   int64_t ncol, double *__restrict__ images,
+                                                   int *__restrict__ bad) {
+    const int s0 = (int)blockIdx.y * S;
+    const int ns = min(S, nshot - s0);
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nn;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        bool oob = false;
+        int ks[S];
+        double m[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            ks[u] = u < ns ? kidx[(int64_t)ts[s0 + u] * nn + q] : 0;
+            m[u] = 0.0;
+        }
+        for (int r = 0; r < nrec; ++r) {
+            const int64_t kr = kidx[(int64_t)tr[r] * nn + q];
+            const double *col = shots + (int64_t)s0 * nt * ncol + (int64_t)xr[r] * nt - 1;
+#pragma unroll
+            for (int u = 0; u < S; ++u) {
+                if (u >= ns) break;
+                const int64_t it = ks[u] + kr;  // 1-based sample
+                if (it < 1 || it > nt) {
+                    oob = true;
+                    continue;
+                }
+                m[u] = m[u] + col[(int64_t)u * nt * ncol + it];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            if (u >= ns) break;
+            images[(int64_t)(s0 + u) * nn + q] = m[u];
+        }
+        if (oob) atomicOr(bad, 1);
+    }
+}
+
+__global__ void __launch_bounds__(256) kirch_sum(const double *__restrict__ images, int64_t nn,
+                                                 int nshot, double *__restrict__ stack) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nn;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        double st = 0.0;
+        for (int s = 0; s < nshot; ++s) st = st + images[(int64_t)s * nn + q];
+        stack[q] = st;
+    }
+}
+
 constexpr int kShotBatch = 4;
 
 // Pass kernel choice: H passes per launch on 64 x TW tiles, or (h = 0) the
@@ -407,12 +465,43 @@ static std::vector<int> receiver_tables(const int *xrec, int nrec, const int *xs
     return tr;
 }
 
+// Device buffers of a migration call: the stack and the per-shot images (or
+// their scratch); a resident handle keeps one across calls, so repeated
+// migrations allocate nothing.
+struct KirchWork {
+    DevBuf<double> stack, img;
+    DevBuf<int> ts, tr, xr, bad;
+    HostBuf<double> hstack;  // pinned (resident handles): the stack comes back at link speed
+    HostBuf<int> hbad;
+    hipEvent_t ev[2] = {nullptr, nullptr};  // kernel timing, created once
+    KirchWork() = default;
+    KirchWork(const KirchWork &) = delete;
+    KirchWork &operator=(const KirchWork &) = delete;
+    ~KirchWork() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    void ensure(int64_t nn, size_t nimg, int nshot, int nrec, bool pinned) {
+        for (hipEvent_t &e : ev)
+            if (!e) SSSI_HIP(hipEventCreate(&e));
+        if (stack.n < (size_t)nn) stack.alloc(nn);
+        if (img.n < nimg) img.alloc(nimg);
+        if (ts.n < (size_t)nshot) ts.alloc(nshot);
+        if (tr.n < (size_t)std::max(nrec, 1)) tr.alloc(std::max(nrec, 1));
+        if (xr.n < (size_t)std::max(nrec, 1)) xr.alloc(std::max(nrec, 1));
+        if (bad.n < 1) bad.alloc(1);
+        if (pinned && hstack.n < (size_t)nn) hstack.alloc(nn);
+        if (pinned && hbad.n < 1) hbad.alloc(1);
+    }
+};
+
 // Migration of nshot device-resident records against resident index tables;
-// returns the kernel time (HIP events on stream s).
+// returns the kernel time (HIP events on stream s).  `work` may be null (the
+// call's own buffers).
 static float kirchhoff_run(const int *kidx_dev, int64_t nz, int64_t nx, int64_t ntab,
                            const int *xrec, int nrec, const int *xsr, int nsr,
                            const double *dshots, int64_t nt, int nshot, const int *ixs,
-                           double *stack, double *images, hipStream_t s) {
+                           double *stack, double *images, hipStream_t s, KirchWork *work) {
     if (nt < 1 || nshot < 1 || !dshots || !ixs || !stack || nrec < 0 || (nrec > 0 && !xrec))
         fail(SSSI_EINVAL, "kirchhoff: bad arguments");
     std::vector<int> tr = receiver_tables(xrec, nrec, xsr, nsr, ntab, nx);
@@ -425,33 +514,49 @@ static float kirchhoff_run(const int *kidx_dev, int64_t nz, int64_t nx, int64_t 
     }
     for (int r = 0; r < nrec; ++r) xr0[r] = xrec[r] - 1;
     const int64_t nn = nz * nx;
-    DevBuf<int> dts(nshot), dtr(std::max(nrec, 1)), dxr(std::max(nrec, 1)), bad(1);
-    DevBuf<double> dstack(nn), dimg(images ? (size_t)nn * nshot : 0);
-    SSSI_HIP(hipMemcpyAsync(dts.p, ts.data(), dts.bytes(), hipMemcpyHostToDevice, s));
+    // shot groups in parallel (then the stack in shot order) when there is
+    // more than one group and their images fit a scratch buffer
+    const int ngroups = (nshot + kShotBatch - 1) / kShotBatch;
+    const bool grouped = ngroups > 1 && ngroups <= 65535 && (double)nn * nshot * 8 <= 4e9;
+    KirchWork local;
+    KirchWork &w = work ? *work : local;
+    const bool pinned = work != nullptr;  // a one-shot call copies straight to the caller
+    w.ensure(nn, images || grouped ? (size_t)nn * nshot : 0, nshot, nrec, pinned);
+    const int *dts = w.ts.p, *dtr = w.tr.p, *dxr = w.xr.p;
+    SSSI_HIP(hipMemcpyAsync(w.ts.p, ts.data(), (size_t)nshot * 4, hipMemcpyHostToDevice, s));
     if (nrec > 0) {
-        SSSI_HIP(hipMemcpyAsync(dtr.p, tr.data(), (size_t)nrec * 4, hipMemcpyHostToDevice, s));
-        SSSI_HIP(hipMemcpyAsync(dxr.p, xr0.data(), (size_t)nrec * 4, hipMemcpyHostToDevice, s));
+        SSSI_HIP(hipMemcpyAsync(w.tr.p, tr.data(), (size_t)nrec * 4, hipMemcpyHostToDevice, s));
+        SSSI_HIP(hipMemcpyAsync(w.xr.p, xr0.data(), (size_t)nrec * 4, hipMemcpyHostToDevice, s));
     }
-    SSSI_HIP(hipMemsetAsync(bad.p, 0, 4, s));
-    hipEvent_t ev[2];
-    SSSI_HIP(hipEventCreate(&ev[0]));
-    SSSI_HIP(hipEventCreate(&ev[1]));
+    SSSI_HIP(hipMemsetAsync(w.bad.p, 0, 4, s));
+    hipEvent_t *ev = w.ev;
     SSSI_HIP(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(kirch_stack<kShotBatch>, dim3(grid_blocks(nn)), dim3(256), 0, s, kidx_dev,
-                       nn, dts.p, nshot, dtr.p, dxr.p, nrec, dshots, nt, nx, dstack.p,
-                       images ? dimg.p : nullptr, bad.p);
+    if (grouped) {
+        hipLaunchKernelGGL(kirch_group<kShotBatch>, dim3(grid_blocks(nn), ngroups), dim3(256), 0,
+                           s, kidx_dev, nn, dts, nshot, dtr, dxr, nrec, dshots, nt, nx,
+                           w.img.p, w.bad.p);
+        hipLaunchKernelGGL(kirch_sum, dim3(grid_blocks(nn)), dim3(256), 0, s, w.img.p, nn, nshot,
+                           w.stack.p);
+    } else {
+        hipLaunchKernelGGL(kirch_stack<kShotBatch>, dim3(grid_blocks(nn)), dim3(256), 0, s,
+                           kidx_dev, nn, dts, nshot, dtr, dxr, nrec, dshots, nt, nx,
+                           w.stack.p, images ? w.img.p : nullptr, w.bad.p);
+    }
     SSSI_HIP(hipGetLastError());
     SSSI_HIP(hipEventRecord(ev[1], s));
-    int hbad = 0;
-    SSSI_HIP(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
-    SSSI_HIP(hipMemcpyAsync(stack, dstack.p, dstack.bytes(), hipMemcpyDeviceToHost, s));
+    int hbad_local = 0;
+    int *hbad_dst = pinned ? w.hbad.p : &hbad_local;
+    SSSI_HIP(hipMemcpyAsync(hbad_dst, w.bad.p, 4, hipMemcpyDeviceToHost, s));
+    SSSI_HIP(hipMemcpyAsync(pinned ? w.hstack.p : stack, w.stack.p, (size_t)nn * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
     if (images)
-        SSSI_HIP(hipMemcpyAsync(images, dimg.p, dimg.bytes(), hipMemcpyDeviceToHost, s));
+        SSSI_HIP(hipMemcpyAsync(images, w.img.p, (size_t)nn * nshot * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
     SSSI_HIP(hipStreamSynchronize(s));
+    if (pinned) std::memcpy(stack, w.hstack.p, (size_t)nn * sizeof(double));
+    const int hbad = *hbad_dst;
     float ms = 0;
     SSSI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
     if (hbad)
         fail(SSSI_EINVAL, "a traveltime sample index falls outside the %lld-sample record "
              "(the reference's migrate indexes past the shot there)", (long long)nt);
@@ -470,6 +575,7 @@ class Kirchhoff2D {
     int64_t rec_nt = 0;
     int rec_shots = 0;
     float migrate_ms = 0;  // kernel time of the last migration
+    KirchWork work;        // stack and image buffers, kept across migrations
 
     void load(const double *shots, int64_t nt, int nshot) {
         if (!shots || nt < 1 || nshot < 1) fail(SSSI_EINVAL, "kirchhoff: no records to load");
@@ -487,7 +593,7 @@ class Kirchhoff2D {
         if (rec_shots < 1) fail(SSSI_ESTATE, "kirchhoff: no records loaded");
         migrate_ms = kirchhoff_run(kidx.p, nz, nx, (int64_t)xsr.size(), xrec, nrec, xsr.data(),
                                    (int)xsr.size(), rec.p, rec_nt, rec_shots, ixs, stack,
-                                   images, eik->stream);
+                                   images, eik->stream, &work);
     }
 
     Kirchhoff2D(const double *v, int64_t nz_, int64_t nx_, double dx, const int *xsr_, int nsr,
@@ -583,7 +689,7 @@ int sssi_kirchhoff_migrate(const double *travelTime, int64_t nz, int64_t nx, int
         DevBuf<double> rec((size_t)nt * nx);
         SSSI_HIP(hipMemcpyAsync(rec.p, shot, rec.bytes(), hipMemcpyHostToDevice, s));
         kirchhoff_run(kidx.p, nz, nx, ntab, xRecGrid, nrec, xShotAndRecGrid, nsr, rec.p, nt, 1,
-                      &ixs, m, nullptr, s);
+                      &ixs, m, nullptr, s, nullptr);
         } catch (...) {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
