@@ -650,7 +650,7 @@ def main():
         out["cfg4_3d"] = _sub(bench_3d, args, steps=2, warmup=1, nt=None)
         out["cfg3_rtm"] = _sub(bench_rtm, args, shots=64)
         out["cfg5_serve"] = _sub(bench_serve, args, jobs=60, rate=20.0)
-        out["kirchhoff"] = _sub(bench_kirchhoff, args, steps=3)
+        out["kirchhoff"] = _sub(bench_kirchhoff, args, steps=20)
         out["freq"] = _sub(bench_freq, args, steps=3)
     print(json.dumps(out), flush=True)
 
