@@ -3,6 +3,8 @@
 
 #include <atomic>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace sssi {
 
@@ -116,6 +118,41 @@ void validate_velocity(const double *v, int64_t n, const char *name) {
     if (q >= 0)
         fail(SSSI_EINVAL, "%s must be finite and positive (element %lld is %g)", name,
              (long long)q, v[q]);
+}
+
+namespace {
+std::mutex g_stream_mu;
+std::map<int, std::vector<hipStream_t>> g_streams;  // idle pool streams per device
+}  // namespace
+
+hipStream_t stream_acquire() {
+    int dev = 0;
+    SSSI_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        auto &v = g_streams[dev];
+        if (!v.empty()) {
+            hipStream_t s = v.back();
+            v.pop_back();
+            return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    SSSI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void stream_release(hipStream_t s) {
+    if (!s) return;
+    int dev = 0;
+    // an idle stream goes back to its own device's list; one that cannot be
+    // drained or placed is destroyed
+    if (hipStreamSynchronize(s) != hipSuccess || hipStreamGetDevice(s, &dev) != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    g_streams[dev].push_back(s);
 }
 
 }  // namespace sssi

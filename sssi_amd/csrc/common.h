@@ -94,18 +94,23 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
-// Non-blocking stream owned by its holder: destroyed with it, also when the
-// holder's constructor throws after creating it.  Declared before the device
-// buffers whose work it orders, it is destroyed after them.
+// Non-blocking streams from a per-device pool.  Creating a stream takes about
+// 3 ms on this runtime, so a handle or a one-shot call takes one from the pool
+// and gives it back (idle) when it is done; the pool keeps them until the
+// process ends.
+hipStream_t stream_acquire();
+void stream_release(hipStream_t s);
+
+// Pool stream held by its holder: given back with it, also when the holder's
+// constructor throws after taking it.  Declared before the device buffers
+// whose work it orders, it is given back after them.
 struct OwnedStream {
     hipStream_t s = nullptr;
     OwnedStream() = default;
     OwnedStream(const OwnedStream &) = delete;
     OwnedStream &operator=(const OwnedStream &) = delete;
-    ~OwnedStream() {
-        if (s) (void)hipStreamDestroy(s);
-    }
-    void create() { SSSI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~OwnedStream() { stream_release(s); }
+    void create() { s = stream_acquire(); }
     operator hipStream_t() const { return s; }
 };
 

@@ -675,9 +675,9 @@ int sssi_kirchhoff_migrate(const double *travelTime, int64_t nz, int64_t nx, int
         if (!travelTime || !xShotAndRecGrid || !m) fail(SSSI_EINVAL, "NULL argument");
         if (nz < 1 || nx < 1 || ntab < 1 || nsr < 1) fail(SSSI_EINVAL, "empty tables");
         if (!(dt > 0) || !std::isfinite(dt)) fail(SSSI_EINVAL, "dt must be positive");
-        hipStream_t s;
-        SSSI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        try {
+        OwnedStream s;
+        s.create();
+        {
             const int64_t n = nz * nx * ntab;
             DevBuf<double> tt(n);
             DevBuf<int> kidx(n);
@@ -690,12 +690,7 @@ int sssi_kirchhoff_migrate(const double *travelTime, int64_t nz, int64_t nx, int
         SSSI_HIP(hipMemcpyAsync(rec.p, shot, rec.bytes(), hipMemcpyHostToDevice, s));
         kirchhoff_run(kidx.p, nz, nx, ntab, xRecGrid, nrec, xShotAndRecGrid, nsr, rec.p, nt, 1,
                       &ixs, m, nullptr, s, nullptr);
-        } catch (...) {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-            throw;
         }
-        (void)hipStreamDestroy(s);
     });
 }
 

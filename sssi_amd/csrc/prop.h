@@ -50,7 +50,7 @@ class Prop {
     // the stream is destroyed here, so that a derived constructor that throws
     // after creating it does not leak it
     virtual ~Prop() {
-        if (stream && own_stream) (void)hipStreamDestroy(stream);
+        if (own_stream) stream_release(stream);
     }
     int dim = 2;
     int abc = SSSI_ABC_CPML;
@@ -118,7 +118,7 @@ class Prop {
     // layout in use: 2-D (tile height, march groups), 3-D (block y extent, z chunk)
     virtual void layout(int *a, int *b) const = 0;
     virtual void use_stream(hipStream_t s) {
-        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        if (own_stream) stream_release(stream);
         stream = s;
         own_stream = false;
     }

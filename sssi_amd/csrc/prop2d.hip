@@ -151,7 +151,7 @@ class Prop2D final : public Prop {
         abc = abc_;
         nz = nz_; nx = nx_; ny = 1; N = N_; nB = nB_; dz = dz_; dx = dx_; dt = dt_;
         SSSI_HIP(hipGetDevice(&device));
-        SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        stream = stream_acquire();
         {
             double vmax = 0.0;
             for (int64_t q = 0; q < nz * nx; ++q) vmax = std::max(vmax, v[q]);

@@ -781,7 +781,7 @@ class Prop3D final : public Prop {
         if (nz * nx * ny > ((int64_t)1 << 31) / 2)
             fail(SSSI_EINVAL, "3-D grid too large (%lld cells)", (long long)(nz * nx * ny));
         SSSI_HIP(hipGetDevice(&device));
-        SSSI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        stream = stream_acquire();
         {
             double vmax = 0.0;
             for (int64_t q = 0; q < nz * nx * ny; ++q) vmax = std::max(vmax, v[q]);
