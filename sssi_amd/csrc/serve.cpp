@@ -138,7 +138,7 @@ class Server {
 
     Server(double budget_bytes, int max_active_) : budget(budget_bytes), max_active(max_active_) {
         SSSI_HIP(hipGetDevice(&device));
-        for (auto &l : lanes) SSSI_HIP(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
+        for (auto &l : lanes) l = stream_acquire();
         // keep freed job memory in the device pool for the next jobs instead of
         // returning it at every synchronisation
         hipMemPool_t pool;
@@ -173,8 +173,7 @@ class Server {
             for (hipEvent_t e : t) (void)hipEventDestroy(e);
         for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
         (void)hipDeviceSynchronize();  // stream-ordered frees above
-        for (auto &l : lanes)
-            if (l) (void)hipStreamDestroy(l);
+        for (auto &l : lanes) stream_release(l);
     }
 
     int64_t submit(std::unique_ptr<ServeJob> j) {
@@ -226,7 +225,7 @@ class Server {
         hipStream_t hs = nullptr;
         try {
             SSSI_HIP(hipSetDevice(device));
-            SSSI_HIP(hipStreamCreateWithFlags(&hs, hipStreamNonBlocking));
+            hs = stream_acquire();
             alloc_stream() = hs;
             while (true) {
                 ServeJob *build = nullptr, *drop = nullptr;
@@ -285,10 +284,7 @@ class Server {
             cv_done.notify_all();
         }
         alloc_stream() = nullptr;
-        if (hs) {
-            (void)hipStreamSynchronize(hs);
-            (void)hipStreamDestroy(hs);
-        }
+        stream_release(hs);  // waits for the stream, then back to the pool
     }
 
     // result and watchdog words to the host (the job's GPU work is complete)
